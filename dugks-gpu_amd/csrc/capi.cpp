@@ -1,0 +1,839 @@
+// C ABI of the engine (include/dugks.h).  Host C++ compiled with hipcc.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/dugks.h"
+#include "dugks_internal.h"
+#include "io.h"
+
+namespace dugks {
+size_t reduction_scratch_doubles();
+}
+
+using namespace dugks;
+
+struct dugks_ctx {
+  Ctx c;
+};
+
+namespace {
+
+thread_local std::string g_init_err;
+
+int fail(dugks_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->c.err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      return fail(ctx, DUGKS_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+#define CHECK_CTX()                        \
+  do {                                     \
+    if (!ctx) return DUGKS_EARG;           \
+    hipSetDevice(ctx->c.device);           \
+  } while (0)
+
+size_t store_bytes(int precision) { return precision == 0 ? 8 : (precision == 1 ? 4 : 2); }
+size_t real_bytes(int precision) { return precision == 0 ? 8 : 4; }
+
+// Weight tables of one axis (layout documented on KParams).
+std::vector<double> axis_table(const Setup& s, int a) {
+  const int n = s.n[a];
+  const double h = s.hdt, dt = s.dt;
+  std::vector<double> t((size_t)(n + 2) * 9, 0.0);
+  const auto& d = s.d[a];
+  for (int m = 0; m <= n + 1; m++) {
+    double* r = &t[(size_t)m * 9];
+    const double dm = d[m];
+    const double dp1 = d[std::min(m + 1, n + 1)], dm1 = d[std::max(m - 1, 0)];
+    for (int c = -1; c <= 1; c++) {
+      if (s.uniform[a]) {
+        // src/dugks.cu:768-772
+        const double dx = s.du[a];
+        const double m1 = (dx / 2. - h * c) / dx;
+        r[c + 1] = m1;
+        r[3 + c + 1] = 1 - m1;
+      } else {
+        // src/fortran/channel/src/dugks.cuf:541, 546
+        r[c + 1] = (dm / 2 - h * c) / (0.5 * dm + 0.5 * dp1);
+        r[3 + c + 1] = (dm / 2 + h * c) / (0.5 * dm + 0.5 * dm1);
+      }
+    }
+    if (s.uniform[a]) {
+      r[6] = std::fabs((-h * -1) / s.du[a]);
+      r[7] = std::fabs((-h * 1) / s.du[a]);
+      r[8] = dt / s.du[a];
+    } else {
+      // src/fortran/channel/src/dugks.cuf:595, 658, 833
+      r[6] = std::fabs((-h * -1) / (0.5 * dm + 0.5 * dp1));
+      r[7] = std::fabs((-h * 1) / (0.5 * dm + 0.5 * dm1));
+      r[8] = dt / dm;
+    }
+  }
+  return t;
+}
+
+int create(const Setup& s, const dugks_opts* opts, dugks_ctx** out) {
+  std::unique_ptr<dugks_ctx> holder(new dugks_ctx);
+  dugks_ctx* ctx = holder.get();
+  Ctx& c = ctx->c;
+  c.s = s;
+  dugks_opts o;
+  dugks_default_opts(&o);
+  if (opts) o = *opts;
+  c.precision = o.precision >= 0 ? o.precision : s.precision;
+  if (c.precision < 0 || c.precision > 2) return fail(nullptr, DUGKS_EARG, "bad precision");
+  c.s.precision = c.precision;
+  c.kernel = o.kernel >= 0 ? o.kernel : s.kernel;
+  c.store_bytes = store_bytes(c.precision);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    g_init_err = "no HIP device available";
+    return DUGKS_EHIP;
+  }
+  if (o.device >= 0) {
+    if (o.device >= ndev) {
+      g_init_err = "device ordinal out of range";
+      return DUGKS_EARG;
+    }
+    c.device = o.device;
+  } else {
+    hipGetDevice(&c.device);
+  }
+  hipSetDevice(c.device);
+  if (o.stream) {
+    c.stream = static_cast<hipStream_t>(o.stream);
+  } else {
+    if (hipStreamCreate(&c.stream) != hipSuccess) {
+      g_init_err = "hipStreamCreate failed";
+      return DUGKS_EHIP;
+    }
+    c.own_stream = true;
+  }
+  if (build_params(c) != hipSuccess) {
+    g_init_err = "device allocation failed: " + c.err;
+    dugks_free(holder.release());
+    return DUGKS_ENOMEM;
+  }
+  const size_t N = c.s.ncell();
+  for (int b = 0; b < 2; b++)
+    if (c.alloc(&c.f[b], N * Q * c.store_bytes) != hipSuccess) {
+      g_init_err = "cannot allocate f (" + std::to_string(N * Q * c.store_bytes) + " bytes)";
+      dugks_free(holder.release());
+      return DUGKS_ENOMEM;
+    }
+  // initial state
+  std::vector<double> f;
+  try {
+    if (c.s.restart) {
+      f.resize(N * Q);
+      read_dump(join_path(c.s.base_dir, dump_name(c.s.nrestart)), f.data(), N * Q, 8);
+    } else {
+      initial_state(c.s, f);
+    }
+  } catch (const ConfigError& e) {
+    g_init_err = e.what();
+    dugks_free(holder.release());
+    return e.code;
+  }
+  int rc = dugks_set_f(ctx, f.data());
+  if (rc != DUGKS_OK) {
+    g_init_err = c.err;
+    dugks_free(holder.release());
+    return rc;
+  }
+  if (c.s.flow_auto) {
+    double sums[8];
+    if ((rc = dugks_global_sums(ctx, sums)) != DUGKS_OK) {
+      g_init_err = c.err;
+      dugks_free(holder.release());
+      return rc;
+    }
+    c.s.flow_rate = sums[4 + c.s.stream_axis()] / sums[7];
+  }
+  if (o.verbose) {
+    std::printf("dugks: grid %d x %d x %d, precision %d, kernel %d, dt=%.9g tau=%.9g\n", s.n[0], s.n[1],
+                s.n[2], c.precision, c.kernel, s.dt, s.tau);
+    std::printf("dugks: bw0=%.9g bw1=%.9g bw2=%.9g fw0=%.9g fw1=%.9g fw2=%.9g\n", s.bw0, s.bw1, s.bw2,
+                s.fw0, s.fw1, s.fw2);
+  }
+  *out = holder.release();
+  return DUGKS_OK;
+}
+
+int ensure_fcell(dugks_ctx* ctx) {
+  Ctx& c = ctx->c;
+  if (!c.fcell) {
+    HIPCHK(c.alloc((void**)&c.fcell, c.s.ncell() * 3 * sizeof(double)));
+    HIPCHK(hipMemsetAsync(c.fcell, 0, c.s.ncell() * 3 * sizeof(double), c.stream));
+  }
+  c.kp.fcell = c.fcell;
+  return DUGKS_OK;
+}
+
+// One step of the current state into the other buffer.
+int step_once(dugks_ctx* ctx, int istep) {
+  Ctx& c = ctx->c;
+  if (c.s.perturb) {
+    int rc = ensure_fcell(ctx);
+    if (rc) return rc;
+    HIPCHK(launch_perturb_force(c, istep));
+  }
+  const void* fin = c.f[c.cur];
+  void* fout = c.f[1 - c.cur];
+  bool handled = false;
+  if (c.kernel != DUGKS_KERNEL_TWOPASS) HIPCHK(launch_step_fused(c, fin, fout, &handled));
+  if (!handled) {
+    if (c.kernel == DUGKS_KERNEL_FUSED)
+      return fail(ctx, DUGKS_EUNSUPPORTED, "the fused kernel does not cover this configuration");
+    if (!c.fbx) {
+      const size_t eN = (size_t)c.kp.eN;
+      HIPCHK(c.alloc(&c.fbx, eN * Q * real_bytes(c.precision)));
+      HIPCHK(hipMemsetAsync(c.fbx, 0, eN * Q * real_bytes(c.precision), c.stream));
+    }
+    HIPCHK(launch_step_twopass(c, fin, fout));
+  }
+  c.cur = 1 - c.cur;
+  if (c.s.flow_rate != 0) HIPCHK(launch_flow_control(c, c.f[c.cur]));
+  return DUGKS_OK;
+}
+
+}  // namespace
+
+namespace dugks {
+
+hipError_t build_params(Ctx& c) {
+  const Setup& s = c.s;
+  KParams& P = c.kp;
+  std::memset(&P, 0, sizeof P);
+  for (int a = 0; a < 3; a++) {
+    P.n[a] = s.n[a];
+    P.periodic[a] = s.periodic[a];
+    P.e[a] = s.n[a] + 2;
+    P.uniform[a] = s.uniform[a];
+    P.du[a] = s.du[a];
+    P.famb[a] = s.famb[a];
+    if (s.periodic[a]) {
+      P.qlo[a] = P.qhi[a] = 0;
+    } else if (s.uniform[a]) {
+      P.qlo[a] = P.qhi[a] = 2.0;  // 2 fb(1) - fb(2), src/dugks.cu:1247-1248
+    } else {
+      const auto& x = s.xc[a];
+      const int n = s.n[a];
+      P.qlo[a] = (x[2] - x[0]) / (x[2] - x[1]);  // dugks.cuf:368
+      P.qhi[a] = (x[n + 1] - x[n - 1]) / (x[n] - x[n - 1]);  // dugks.cuf:372
+    }
+  }
+  P.N = (long long)s.n[0] * s.n[1] * s.n[2];
+  P.eN = (long long)P.e[0] * P.e[1] * P.e[2];
+  P.dt = s.dt;
+  P.hdt = s.hdt;
+  P.bw0 = s.bw0;
+  P.bw1 = s.bw1;
+  P.bw2 = s.bw2;
+  P.fw0 = s.fw0;
+  P.fw1 = s.fw1;
+  P.fw2 = s.fw2;
+  P.recon = s.recon;
+  for (int ip = 0; ip < Q; ip++) {
+    P.hbase[ip] = (float)(wgt(ip) * s.rho0);
+    P.hw[ip] = (float)wgt(ip);
+    P.hinvw[ip] = (float)(1.0 / wgt(ip));
+  }
+  hipError_t e;
+  for (int a = 0; a < 3; a++) {
+    std::vector<double> t = axis_table(s, a);
+    if ((e = c.alloc((void**)&c.tab[a], t.size() * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMemcpy(c.tab[a], t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+      return e;
+    P.tab[a] = c.tab[a];
+    if ((e = c.alloc((void**)&c.wd[a], s.d[a].size() * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMemcpy(c.wd[a], s.d[a].data(), s.d[a].size() * sizeof(double), hipMemcpyHostToDevice)) !=
+        hipSuccess)
+      return e;
+    P.wd[a] = c.wd[a];
+  }
+  if ((e = c.alloc((void**)&c.famb_dev, 8 * sizeof(double))) != hipSuccess) return e;
+  double fa[8] = {s.famb[0], s.famb[1], s.famb[2], 0, 0, 0, 0, 0};
+  if ((e = hipMemcpy(c.famb_dev, fa, sizeof fa, hipMemcpyHostToDevice)) != hipSuccess) return e;
+  P.famb_dev = c.famb_dev;
+  c.red_len = reduction_scratch_doubles() + 64;
+  if ((e = c.alloc((void**)&c.red, c.red_len * sizeof(double))) != hipSuccess) return e;
+  return hipSuccess;
+}
+
+}  // namespace dugks
+
+namespace {
+template <class F>
+int host_guard(F&& f) {
+  g_init_err.clear();
+  try {
+    return f();
+  } catch (const ConfigError& e) {
+    g_init_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_init_err = e.what();
+    return DUGKS_ECONFIG;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+void dugks_default_opts(dugks_opts* o) {
+  if (!o) return;
+  o->precision = DUGKS_PREC_CONFIG;
+  o->device = -1;
+  o->stream = nullptr;
+  o->kernel = DUGKS_KERNEL_AUTO;
+  o->verbose = 0;
+}
+
+const char* dugks_init_error(void) { return g_init_err.c_str(); }
+
+int dugks_init_from_string(const char* text, const char* base_dir, const dugks_opts* opts,
+                           dugks_ctx** out) {
+  g_init_err.clear();
+  if (!text || !out) {
+    g_init_err = "null argument";
+    return DUGKS_EARG;
+  }
+  *out = nullptr;
+  try {
+    Config cfg = Config::from_text(text);
+    Setup s = make_setup(cfg, base_dir ? base_dir : "");
+    return create(s, opts, out);
+  } catch (const ConfigError& e) {
+    g_init_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_init_err = e.what();
+    return DUGKS_ECONFIG;
+  }
+}
+
+int dugks_init(const char* config_path, const dugks_opts* opts, dugks_ctx** out) {
+  g_init_err.clear();
+  if (!config_path || !out) {
+    g_init_err = "null argument";
+    return DUGKS_EARG;
+  }
+  *out = nullptr;
+  try {
+    Config cfg = Config::from_file(config_path);
+    std::string p(config_path);
+    size_t sl = p.find_last_of('/');
+    std::string dir = sl == std::string::npos ? "" : p.substr(0, sl);
+    Setup s = make_setup(cfg, dir);
+    return create(s, opts, out);
+  } catch (const ConfigError& e) {
+    g_init_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_init_err = e.what();
+    return DUGKS_ECONFIG;
+  }
+}
+
+void dugks_free(dugks_ctx* ctx) {
+  if (!ctx) return;
+  Ctx& c = ctx->c;
+  hipSetDevice(c.device);
+  if (c.stream) hipStreamSynchronize(c.stream);
+  for (int b = 0; b < 2; b++) hipFree(c.f[b]);
+  hipFree(c.fbx);
+  for (int a = 0; a < 3; a++) {
+    hipFree(c.tab[a]);
+    hipFree(c.wd[a]);
+  }
+  hipFree(c.famb_dev);
+  hipFree(c.fcell);
+  hipFree(c.red);
+  for (int q = 0; q < 2; q++)
+    if (c.tev[q]) hipEventDestroy(c.tev[q]);
+  if (c.own_stream && c.stream) hipStreamDestroy(c.stream);
+  delete ctx;
+}
+
+const char* dugks_last_error(const dugks_ctx* ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+
+int dugks_step(dugks_ctx* ctx, int istep0, int nsteps) {
+  CHECK_CTX();
+  if (nsteps < 0) return fail(ctx, DUGKS_EARG, "nsteps < 0");
+  for (int s = 0; s < nsteps; s++) {
+    int rc = step_once(ctx, istep0 + s);
+    if (rc) return rc;
+  }
+  return DUGKS_OK;
+}
+
+int dugks_time_steps(dugks_ctx* ctx, int istep0, int nsteps, double out[5]) {
+  CHECK_CTX();
+  if (nsteps <= 0 || !out) return fail(ctx, DUGKS_EARG, "bad timing arguments");
+  Ctx& c = ctx->c;
+  hipEvent_t all[2];
+  for (int q = 0; q < 2; q++) {
+    if (!c.tev[q]) HIPCHK(hipEventCreate(&c.tev[q]));
+    HIPCHK(hipEventCreate(&all[q]));
+  }
+  double sum = 0, mn = 1e300, mx = 0;
+  HIPCHK(hipEventRecord(all[0], c.stream));
+  c.timing = true;
+  int rc = DUGKS_OK;
+  for (int s = 0; s < nsteps && rc == DUGKS_OK; s++) {
+    rc = step_once(ctx, istep0 + s);
+    if (rc) break;
+    hipError_t e = hipEventSynchronize(c.tev[1]);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c.tev[0], c.tev[1]);
+    if (e != hipSuccess) rc = fail(ctx, DUGKS_EHIP, std::string("timing: ") + hipGetErrorString(e));
+    sum += ms;
+    mn = std::min(mn, (double)ms);
+    mx = std::max(mx, (double)ms);
+  }
+  c.timing = false;
+  float total = 0;
+  if (rc == DUGKS_OK) {
+    HIPCHK(hipEventRecord(all[1], c.stream));
+    HIPCHK(hipEventSynchronize(all[1]));
+    HIPCHK(hipEventElapsedTime(&total, all[0], all[1]));
+  }
+  hipEventDestroy(all[0]);
+  hipEventDestroy(all[1]);
+  if (rc) return rc;
+  out[0] = sum / nsteps;
+  out[1] = mn;
+  out[2] = mx;
+  out[3] = total;
+  out[4] = c.launches_per_step;
+  return DUGKS_OK;
+}
+
+int dugks_sync(dugks_ctx* ctx) {
+  CHECK_CTX();
+  HIPCHK(hipStreamSynchronize(ctx->c.stream));
+  return DUGKS_OK;
+}
+
+int dugks_get_dims(const dugks_ctx* ctx, int dims[3]) {
+  if (!ctx || !dims) return DUGKS_EARG;
+  for (int a = 0; a < 3; a++) dims[a] = ctx->c.s.n[a];
+  return DUGKS_OK;
+}
+
+int dugks_get_params(const dugks_ctx* ctx, double out[16]) {
+  if (!ctx || !out) return DUGKS_EARG;
+  const Setup& s = ctx->c.s;
+  double mins[3];
+  for (int a = 0; a < 3; a++) {
+    mins[a] = s.du[a];
+    if (!s.uniform[a])
+      for (int m = 1; m <= s.n[a]; m++) mins[a] = std::min(mins[a], s.d[a][m]);
+  }
+  double v[16] = {s.dt,  s.hdt, s.tau, s.visc, s.bw0,   s.bw1,   s.bw2,   s.fw0,
+                  s.fw1, s.fw2, mins[0], mins[1], mins[2], s.rho0, (double)ctx->c.precision,
+                  (double)ctx->c.kernel};
+  std::memcpy(out, v, sizeof v);
+  return DUGKS_OK;
+}
+
+int dugks_get_schedule(const dugks_ctx* ctx, int out[6]) {
+  if (!ctx || !out) return DUGKS_EARG;
+  const Setup& s = ctx->c.s;
+  int v[6] = {s.istep0, s.nsteps, s.noutput2, s.noutput3, s.ndump, s.nprint};
+  std::memcpy(out, v, sizeof v);
+  return DUGKS_OK;
+}
+
+int dugks_write_outputs(dugks_ctx* ctx, int step, const char* dir) {
+  CHECK_CTX();
+  const size_t N = ctx->c.s.ncell();
+  std::vector<double> m(4 * N);
+  int rc = dugks_get_macro(ctx, m.data(), m.data() + N, m.data() + 2 * N, m.data() + 3 * N);
+  if (rc) return rc;
+  try {
+    write_outputs(ctx->c.s, step, m.data(), m.data() + N, m.data() + 2 * N, m.data() + 3 * N, dir ? dir : "");
+  } catch (const ConfigError& e) {
+    return fail(ctx, e.code, e.what());
+  }
+  return DUGKS_OK;
+}
+
+int dugks_get_spacing(const dugks_ctx* ctx, int axis, double* widths) {
+  if (!ctx || !widths || axis < 0 || axis > 2) return DUGKS_EARG;
+  const Setup& s = ctx->c.s;
+  for (int m = 1; m <= s.n[axis]; m++) widths[m - 1] = s.d[axis][m];
+  return DUGKS_OK;
+}
+
+int dugks_get_f_device(dugks_ctx* ctx, double* dev_f) {
+  CHECK_CTX();
+  if (!dev_f) return fail(ctx, DUGKS_EARG, "null device pointer");
+  HIPCHK(launch_export(ctx->c, ctx->c.f[ctx->c.cur], dev_f));
+  return DUGKS_OK;
+}
+
+int dugks_set_f_device(dugks_ctx* ctx, const double* dev_f) {
+  CHECK_CTX();
+  if (!dev_f) return fail(ctx, DUGKS_EARG, "null device pointer");
+  HIPCHK(launch_import(ctx->c, dev_f, ctx->c.f[ctx->c.cur]));
+  return DUGKS_OK;
+}
+
+int dugks_get_f(dugks_ctx* ctx, double* host_f) {
+  CHECK_CTX();
+  if (!host_f) return fail(ctx, DUGKS_EARG, "null host pointer");
+  Ctx& c = ctx->c;
+  const size_t N = c.s.ncell();
+  double* tmp = nullptr;
+  HIPCHK(hipMalloc((void**)&tmp, N * Q * sizeof(double)));
+  hipError_t e = launch_export(c, c.f[c.cur], tmp);
+  std::vector<double> soa;
+  if (e == hipSuccess) {
+    soa.resize(N * Q);
+    e = hipMemcpyAsync(soa.data(), tmp, N * Q * sizeof(double), hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  }
+  hipFree(tmp);
+  if (e != hipSuccess) return fail(ctx, DUGKS_EHIP, std::string("get_f: ") + hipGetErrorString(e));
+  for (size_t cell = 0; cell < N; cell++)
+    for (int ip = 0; ip < Q; ip++) host_f[cell * Q + ip] = soa[(size_t)ip * N + cell];
+  return DUGKS_OK;
+}
+
+int dugks_set_f(dugks_ctx* ctx, const double* host_f) {
+  CHECK_CTX();
+  if (!host_f) return fail(ctx, DUGKS_EARG, "null host pointer");
+  Ctx& c = ctx->c;
+  const size_t N = c.s.ncell();
+  std::vector<double> soa(N * Q);
+  for (size_t cell = 0; cell < N; cell++)
+    for (int ip = 0; ip < Q; ip++) {
+      const double v = host_f[cell * Q + ip];
+      if (!std::isfinite(v)) return fail(ctx, DUGKS_ENUMERIC, "non-finite value in f");
+      soa[(size_t)ip * N + cell] = v;
+    }
+  double* tmp = nullptr;
+  HIPCHK(hipMalloc((void**)&tmp, N * Q * sizeof(double)));
+  hipError_t e = hipMemcpyAsync(tmp, soa.data(), N * Q * sizeof(double), hipMemcpyHostToDevice, c.stream);
+  if (e == hipSuccess) e = launch_import(c, tmp, c.f[c.cur]);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  hipFree(tmp);
+  if (e != hipSuccess) return fail(ctx, DUGKS_EHIP, std::string("set_f: ") + hipGetErrorString(e));
+  // the flow-rate controller restarts from the new state
+  double z[2] = {0, 0};
+  HIPCHK(hipMemcpy(c.famb_dev + 3, z, sizeof z, hipMemcpyHostToDevice));
+  return DUGKS_OK;
+}
+
+int dugks_get_macro(dugks_ctx* ctx, double* rho, double* ux, double* uy, double* uz) {
+  CHECK_CTX();
+  if (!rho || !ux || !uy || !uz) return fail(ctx, DUGKS_EARG, "null host pointer");
+  Ctx& c = ctx->c;
+  const size_t N = c.s.ncell();
+  double* tmp = nullptr;
+  HIPCHK(hipMalloc((void**)&tmp, 4 * N * sizeof(double)));
+  hipError_t e = launch_macro(c, c.f[c.cur], tmp, tmp + N, tmp + 2 * N, tmp + 3 * N);
+  double* dst[4] = {rho, ux, uy, uz};
+  for (int q = 0; q < 4 && e == hipSuccess; q++)
+    e = hipMemcpyAsync(dst[q], tmp + q * N, N * sizeof(double), hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  hipFree(tmp);
+  if (e != hipSuccess) return fail(ctx, DUGKS_EHIP, std::string("get_macro: ") + hipGetErrorString(e));
+  return DUGKS_OK;
+}
+
+int dugks_set_ambient_force(dugks_ctx* ctx, const double force[3]) {
+  CHECK_CTX();
+  if (!force) return fail(ctx, DUGKS_EARG, "null force");
+  Ctx& c = ctx->c;
+  for (int a = 0; a < 3; a++) c.s.famb[a] = c.kp.famb[a] = force[a];
+  double fa[5] = {force[0], force[1], force[2], 0, 0};
+  HIPCHK(hipMemcpyAsync(c.famb_dev, fa, sizeof fa, hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return DUGKS_OK;
+}
+
+int dugks_get_ambient_force(dugks_ctx* ctx, double force[3]) {
+  CHECK_CTX();
+  if (!force) return fail(ctx, DUGKS_EARG, "null force");
+  HIPCHK(hipMemcpyAsync(force, ctx->c.famb_dev, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->c.stream));
+  HIPCHK(hipStreamSynchronize(ctx->c.stream));
+  return DUGKS_OK;
+}
+
+int dugks_set_cell_force(dugks_ctx* ctx, const double* fx, const double* fy, const double* fz) {
+  CHECK_CTX();
+  Ctx& c = ctx->c;
+  if (!fx && !fy && !fz) {
+    c.kp.fcell = nullptr;
+    return DUGKS_OK;
+  }
+  if (!fx || !fy || !fz) return fail(ctx, DUGKS_EARG, "give all three components or none");
+  int rc = ensure_fcell(ctx);
+  if (rc) return rc;
+  const size_t N = c.s.ncell();
+  const double* src[3] = {fx, fy, fz};
+  for (int a = 0; a < 3; a++)
+    HIPCHK(hipMemcpyAsync(c.fcell + a * N, src[a], N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return DUGKS_OK;
+}
+
+int dugks_set_cell_force_device(dugks_ctx* ctx, const double* dev_force) {
+  CHECK_CTX();
+  Ctx& c = ctx->c;
+  if (!dev_force) {
+    c.kp.fcell = nullptr;
+    return DUGKS_OK;
+  }
+  int rc = ensure_fcell(ctx);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(c.fcell, dev_force, 3 * c.s.ncell() * sizeof(double), hipMemcpyDeviceToDevice,
+                        c.stream));
+  return DUGKS_OK;
+}
+
+int dugks_write_dump(dugks_ctx* ctx, int step, const char* dir, int rb) {
+  CHECK_CTX();
+  if (rb != 4 && rb != 8) return fail(ctx, DUGKS_EARG, "real_bytes must be 4 or 8");
+  const size_t count = ctx->c.s.ncell() * Q;
+  std::vector<double> f(count);
+  int rc = dugks_get_f(ctx, f.data());
+  if (rc) return rc;
+  try {
+    write_dump(join_path(dir ? dir : "", dump_name(step)), f.data(), count, rb);
+  } catch (const ConfigError& e) {
+    return fail(ctx, e.code, e.what());
+  }
+  return DUGKS_OK;
+}
+
+int dugks_read_dump(dugks_ctx* ctx, int step, const char* dir, int rb) {
+  CHECK_CTX();
+  if (rb != 4 && rb != 8) return fail(ctx, DUGKS_EARG, "real_bytes must be 4 or 8");
+  const size_t count = ctx->c.s.ncell() * Q;
+  std::vector<double> f(count);
+  try {
+    read_dump(join_path(dir ? dir : "", dump_name(step)), f.data(), count, rb);
+  } catch (const ConfigError& e) {
+    return fail(ctx, e.code, e.what());
+  }
+  return dugks_set_f(ctx, f.data());
+}
+
+int dugks_global_sums(dugks_ctx* ctx, double out[8]) {
+  CHECK_CTX();
+  if (!out) return fail(ctx, DUGKS_EARG, "null output");
+  Ctx& c = ctx->c;
+  double* dev = c.red + reduction_scratch_doubles();
+  HIPCHK(launch_global_sums(c, c.f[c.cur], dev));
+  HIPCHK(hipMemcpyAsync(out, dev, 8 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return DUGKS_OK;
+}
+
+int dugks_loss(dugks_ctx* ctx, int kind, double* value, double* dev_grad) {
+  CHECK_CTX();
+  if (!value || (kind != 0 && kind != 1)) return fail(ctx, DUGKS_EARG, "bad loss arguments");
+  Ctx& c = ctx->c;
+  if (kind == 0 && c.s.wall_axis() < 0) return fail(ctx, DUGKS_EUNSUPPORTED, "wall shear needs a wall axis");
+  double* dev = c.red + reduction_scratch_doubles() + 8;
+  HIPCHK(launch_loss(c, c.f[c.cur], kind, dev, dev_grad));
+  HIPCHK(hipMemcpyAsync(value, dev, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return DUGKS_OK;
+}
+
+int dugks_adjoint_step(dugks_ctx* ctx, const double* lam_next, double* lam_prev, double* grad_force) {
+  CHECK_CTX();
+  if (!lam_next || !lam_prev) return fail(ctx, DUGKS_EARG, "null adjoint buffer");
+  Ctx& c = ctx->c;
+  HIPCHK(launch_adjoint(c, c.f[c.cur], lam_next, lam_prev, grad_force));
+  return DUGKS_OK;
+}
+
+int dugks_forward_adjoint(dugks_ctx* ctx, int istep0, int T, int loss_kind, int interval, double* J,
+                          double* dev_grad_force, double* dev_grad_f0, double* peak_bytes) {
+  CHECK_CTX();
+  if (T <= 0 || !J || !dev_grad_force || (loss_kind != 0 && loss_kind != 1))
+    return fail(ctx, DUGKS_EARG, "bad forward_adjoint arguments");
+  Ctx& c = ctx->c;
+  if (loss_kind == 0 && c.s.wall_axis() < 0) return fail(ctx, DUGKS_EUNSUPPORTED, "wall shear needs a wall axis");
+  if (c.s.flow_rate != 0 || c.s.perturb)
+    return fail(ctx, DUGKS_EUNSUPPORTED, "adjoint with flow-rate control or perturb forcing is not supported");
+  const size_t N = c.s.ncell();
+  const size_t nf = N * Q;
+  if (interval <= 0) interval = (int)std::ceil(std::sqrt((double)T));
+  const int ncp = (T + interval - 1) / interval;  // checkpoints at steps 0, K, 2K, ...
+  const size_t sb = c.store_bytes;
+  // checkpoint store (state at the start of each segment) + segment states
+  std::vector<void*> cps(ncp, nullptr), seg(interval, nullptr);
+  double *lam = nullptr, *lam2 = nullptr, *g = nullptr, *dJ = nullptr;
+  double bytes = 0;
+  auto cleanup = [&]() {
+    for (void* p : cps) hipFree(p);
+    for (void* p : seg) hipFree(p);
+    hipFree(lam);
+    hipFree(lam2);
+    hipFree(g);
+    hipFree(dJ);
+  };
+  auto chk = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+      cleanup();
+      return fail(ctx, e == hipErrorOutOfMemory ? DUGKS_ENOMEM : DUGKS_EHIP,
+                  std::string(what) + ": " + hipGetErrorString(e));
+    }
+    return DUGKS_OK;
+  };
+  int rc;
+  for (int q = 0; q < ncp; q++) {
+    if ((rc = chk(hipMalloc(&cps[q], nf * sb), "checkpoint alloc"))) return rc;
+    bytes += (double)nf * sb;
+  }
+  for (int q = 0; q < interval; q++) {
+    if ((rc = chk(hipMalloc(&seg[q], nf * sb), "segment alloc"))) return rc;
+    bytes += (double)nf * sb;
+  }
+  if ((rc = chk(hipMalloc((void**)&lam, nf * sizeof(double)), "lambda alloc"))) return rc;
+  if ((rc = chk(hipMalloc((void**)&lam2, nf * sizeof(double)), "lambda alloc"))) return rc;
+  if ((rc = chk(hipMalloc((void**)&dJ, nf * sizeof(double)), "lambda alloc"))) return rc;
+  bytes += 3.0 * nf * sizeof(double);
+  double* dval = c.red + reduction_scratch_doubles() + 9;
+  // forward sweep: checkpoints and the loss
+  double Jsum = 0;
+  for (int n = 0; n < T; n++) {
+    if (n % interval == 0)
+      if ((rc = chk(hipMemcpyAsync(cps[n / interval], c.f[c.cur], nf * sb, hipMemcpyDeviceToDevice, c.stream),
+                    "checkpoint copy")))
+        return rc;
+    if ((rc = step_once(ctx, istep0 + n))) {
+      cleanup();
+      return rc;
+    }
+    double v = 0;
+    if ((rc = chk(launch_loss(c, c.f[c.cur], loss_kind, dval, nullptr), "loss"))) return rc;
+    if ((rc = chk(hipMemcpyAsync(&v, dval, sizeof(double), hipMemcpyDeviceToHost, c.stream), "loss copy")))
+      return rc;
+    if ((rc = chk(hipStreamSynchronize(c.stream), "sync"))) return rc;
+    Jsum += v;
+  }
+  *J = Jsum / T;
+  const void* fT = c.f[c.cur];  // final state; the context keeps it
+  // reverse sweep
+  if ((rc = chk(hipMemsetAsync(lam, 0, nf * sizeof(double), c.stream), "memset"))) return rc;
+  if ((rc = chk(hipMemsetAsync(dev_grad_force, 0, 3 * N * sizeof(double), c.stream), "memset"))) return rc;
+  const double invT = 1.0 / T;
+  // Recompute segments from their checkpoints into seg[], then step back.
+  for (int q = ncp - 1; q >= 0; q--) {
+    const int n0 = q * interval, n1 = std::min(T, n0 + interval);
+    // seg[s] = f^{n0+s}
+    if ((rc = chk(hipMemcpyAsync(seg[0], cps[q], nf * sb, hipMemcpyDeviceToDevice, c.stream), "seg copy")))
+      return rc;
+    for (int s = 1; s < n1 - n0; s++) {
+      bool handled = false;
+      hipError_t e = hipSuccess;
+      if (c.kernel != DUGKS_KERNEL_TWOPASS) e = launch_step_fused(c, seg[s - 1], seg[s], &handled);
+      if (e == hipSuccess && !handled) e = launch_step_twopass(c, seg[s - 1], seg[s]);
+      if ((rc = chk(e, "segment recompute"))) return rc;
+    }
+    for (int n = n1 - 1; n >= n0; n--) {
+      // lam currently = dJ/df^{n+1}; add the loss term at step n+1
+      const void* fnext = (n + 1 < n1) ? seg[n + 1 - n0] : (n1 == T ? fT : cps[q + 1]);
+      if ((rc = chk(launch_loss(c, fnext, loss_kind, dval, dJ), "loss grad"))) return rc;
+      if ((rc = chk(launch_axpy(c, invT, dJ, lam, (long long)nf), "axpy"))) return rc;
+      // lam2 = (df^{n+1}/df^n)^T lam, about f^n = seg[n - n0]
+      if ((rc = chk(launch_adjoint(c, seg[n - n0], lam, lam2, dev_grad_force), "adjoint"))) return rc;
+      std::swap(lam, lam2);
+    }
+  }
+  if (dev_grad_f0)
+    if ((rc = chk(hipMemcpyAsync(dev_grad_f0, lam, nf * sizeof(double), hipMemcpyDeviceToDevice, c.stream),
+                  "grad copy")))
+      return rc;
+  if ((rc = chk(hipStreamSynchronize(c.stream), "sync"))) return rc;
+  if (peak_bytes) *peak_bytes = bytes;
+  cleanup();
+  return DUGKS_OK;
+}
+
+// ---- host-only helpers ----
+
+
+int dugks_config_derive(const char* text, const char* base_dir, int dims[3], double params[16]) {
+  if (!text || !dims || !params) return DUGKS_EARG;
+  return host_guard([&]() {
+    Setup s = make_setup(Config::from_text(text), base_dir ? base_dir : "");
+    double mins[3];
+    for (int a = 0; a < 3; a++) {
+      dims[a] = s.n[a];
+      mins[a] = s.du[a];
+      if (!s.uniform[a])
+        for (int m = 1; m <= s.n[a]; m++) mins[a] = std::min(mins[a], s.d[a][m]);
+    }
+    double v[16] = {s.dt,  s.hdt, s.tau,   s.visc,  s.bw0,   s.bw1,  s.bw2,          s.fw0,
+                    s.fw1, s.fw2, mins[0], mins[1], mins[2], s.rho0, (double)s.precision, (double)s.kernel};
+    std::memcpy(params, v, sizeof v);
+    return DUGKS_OK;
+  });
+}
+
+int dugks_config_spacing(const char* text, const char* base_dir, int axis, double* widths) {
+  if (!text || !widths || axis < 0 || axis > 2) return DUGKS_EARG;
+  return host_guard([&]() {
+    Setup s = make_setup(Config::from_text(text), base_dir ? base_dir : "");
+    for (int m = 1; m <= s.n[axis]; m++) widths[m - 1] = s.d[axis][m];
+    return DUGKS_OK;
+  });
+}
+
+int dugks_config_initial_state(const char* text, const char* base_dir, double* host_f) {
+  if (!text || !host_f) return DUGKS_EARG;
+  return host_guard([&]() {
+    Setup s = make_setup(Config::from_text(text), base_dir ? base_dir : "");
+    if (s.restart) throw ConfigError(DUGKS_ESTATE, "restart=1: the initial state comes from a dump");
+    std::vector<double> f;
+    initial_state(s, f);
+    std::memcpy(host_f, f.data(), f.size() * sizeof(double));
+    return DUGKS_OK;
+  });
+}
+
+int dugks_mesh_tanh(int n, double alpha, double width, double* faces) {
+  if (n < 1 || !(alpha > 0 && alpha < 1) || !(width > 0) || !faces) return DUGKS_EARG;
+  std::vector<double> z = tanh_faces(n, alpha, width);
+  std::memcpy(faces, z.data(), z.size() * sizeof(double));
+  return DUGKS_OK;
+}
+
+int dugks_dump_write(const char* path, const double* f, long long count, int rb) {
+  if (!path || !f || count < 0) return DUGKS_EARG;
+  return host_guard([&]() {
+    write_dump(path, f, (size_t)count, rb);
+    return DUGKS_OK;
+  });
+}
+
+int dugks_dump_read(const char* path, double* f, long long count, int rb) {
+  if (!path || !f || count < 0) return DUGKS_EARG;
+  return host_guard([&]() {
+    read_dump(path, f, (size_t)count, rb);
+    return DUGKS_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,102 @@
+// Device helpers shared by the step, adjoint and utility kernels.
+#pragma once
+
+#include <type_traits>
+#include <utility>
+
+#include "dugks_internal.h"
+
+namespace dugks {
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I = 0..N-1, so that
+// lattice components are constants in every unrolled body.
+template <class F, int... Is>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, Is...>) {
+  (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// Storage codecs: Store is the element type of f in HBM, Real the
+// arithmetic type.
+template <class Store>
+struct Codec;
+
+template <>
+struct Codec<double> {
+  template <class Real>
+  __device__ __forceinline__ static Real load(const double* p, long long i, const KParams&, int) {
+    return (Real)p[i];
+  }
+  template <class Real>
+  __device__ __forceinline__ static void store(double* p, long long i, Real v, const KParams&, int) {
+    p[i] = (double)v;
+  }
+};
+
+template <>
+struct Codec<float> {
+  template <class Real>
+  __device__ __forceinline__ static Real load(const float* p, long long i, const KParams&, int) {
+    return (Real)p[i];
+  }
+  template <class Real>
+  __device__ __forceinline__ static void store(float* p, long long i, Real v, const KParams&, int) {
+    p[i] = (float)v;
+  }
+};
+
+// FP16 deviation storage: s_i = (f_i - w_i rho0) / w_i, f_i = w_i rho0 + w_i s_i.
+template <>
+struct Codec<__half> {
+  template <class Real>
+  __device__ __forceinline__ static Real load(const __half* p, long long i, const KParams& P, int ip) {
+    return (Real)(P.hbase[ip] + P.hw[ip] * __half2float(p[i]));
+  }
+  template <class Real>
+  __device__ __forceinline__ static void store(__half* p, long long i, Real v, const KParams& P, int ip) {
+    p[i] = __float2half(((float)v - P.hbase[ip]) * P.hinvw[ip]);
+  }
+};
+
+template <class Real>
+__device__ __forceinline__ Real rcp_real(Real x) {
+  return Real(1) / x;
+}
+
+// Centre-stage f-bar+ of one cell (calc_fb1 of the reference, src/dugks.cu:1131-1196):
+//   rho = sum f, u = j/rho + h F, feq_i(rho, u),
+//   fb_i = bw1 f_i + (bw2 + bw0 (c_i - u).F / RT) feq_i.
+// Written with the +/- pairing of opposite populations, which only reorders
+// the arithmetic.
+template <class Real>
+__device__ __forceinline__ void centre_fbar(const Real (&f)[Q], const Real F[3], Real hdt, Real bw0,
+                                            Real bw1, Real bw2, Real (&fb)[Q]) {
+  Real rho = 0, jx = 0, jy = 0, jz = 0;
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    rho += f[ip];
+    if constexpr (CX[ip] != 0) jx += CX[ip] > 0 ? f[ip] : -f[ip];
+    if constexpr (CY[ip] != 0) jy += CY[ip] > 0 ? f[ip] : -f[ip];
+    if constexpr (CZ[ip] != 0) jz += CZ[ip] > 0 ? f[ip] : -f[ip];
+  });
+  const Real inv = rcp_real(rho);
+  const Real ux = jx * inv + hdt * F[0];
+  const Real uy = jy * inv + hdt * F[1];
+  const Real uz = jz * inv + hdt * F[2];
+  const Real uu = ux * ux + uy * uy + uz * uz;
+  const Real base = Real(1) - Real(1.5) * uu;  // 1 - u.u/(2RT)
+  const Real uF = ux * F[0] + uy * F[1] + uz * F[2];
+  const Real P0 = bw2 - Real(3) * bw0 * uF;    // bw2 + bw0 (-u.F)/RT
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    const Real cu = Real(CX[ip]) * ux + Real(CY[ip]) * uy + Real(CZ[ip]) * uz;
+    const Real cF = Real(CX[ip]) * F[0] + Real(CY[ip]) * F[1] + Real(CZ[ip]) * F[2];
+    const Real feq = Real(wgt(ip)) * rho * (base + Real(3) * cu + Real(4.5) * cu * cu);
+    fb[ip] = bw1 * f[ip] + (P0 + Real(3) * bw0 * cF) * feq;
+  });
+}
+
+}  // namespace dugks

@@ -1,0 +1,104 @@
+// Internal declarations shared by the HIP kernels and the C ABI.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "lattice.h"
+#include "setup.h"
+
+namespace dugks {
+
+// Kernel parameters, passed by value to every step kernel.
+//
+// Device storage of f: structure of arrays f[ip][k][j][i] (direction stride
+// N = nx*ny*nz), interior cells only; ghost cells are never stored, the
+// kernels apply the periodic wrap / wall extrapolation on the fly.
+//
+// Per-axis weight tables tab[a], 9 doubles per ghost-inclusive index m =
+// 0..n+1 (m = 1 is the first interior cell):
+//   [0..2] fraction toward m+1 at the +face of cell m for c_a = -1, 0, +1
+//   [3..5] fraction toward m-1 at the -face of cell m for c_a = -1, 0, +1
+//   [6]    tangential fraction toward m+1 (population with c_a = -1)
+//   [7]    tangential fraction toward m-1 (population with c_a = +1)
+//   [8]    dt / width(m)   (update coefficient)
+struct KParams {
+  int n[3];
+  int periodic[3];
+  long long N;    // nx*ny*nz
+  int e[3];       // n+2
+  long long eN;   // ghost-inclusive cell count
+  double dt, hdt, bw0, bw1, bw2, fw0, fw1, fw2;
+  double famb[3];
+  const double* famb_dev;  // device ambient force (3 doubles), overrides famb when set
+  const double* fcell;     // additive centre force [3][N], or nullptr
+  const double* tab[3];
+  const double* wd[3];     // ghost-inclusive cell widths, index 0..n+1
+  double qlo[3], qhi[3];   // wall ghost = q*f(1) + (1-q)*f(2) (and mirrored at the top)
+  int uniform[3];
+  double du[3];
+  int recon;
+  // FP16 deviation codec: f_i = base_i + w_i * s_i, s_i = (f_i - base_i) / w_i
+  float hbase[Q];
+  float hw[Q];
+  float hinvw[Q];
+};
+
+struct Ctx;
+
+// Launchers (implemented in the .hip files; all stream-ordered).
+hipError_t launch_step_twopass(Ctx& c, const void* fin, void* fout);
+hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled);
+hipError_t launch_macro(Ctx& c, const void* f, double* rho, double* ux, double* uy, double* uz);
+hipError_t launch_export(Ctx& c, const void* f, double* dst);
+hipError_t launch_import(Ctx& c, const double* src, void* f);
+hipError_t launch_global_sums(Ctx& c, const void* f, double* dev_out8);
+hipError_t launch_flow_control(Ctx& c, const void* f);
+hipError_t launch_perturb_force(Ctx& c, int step);
+hipError_t launch_adjoint(Ctx& c, const void* f, const double* lam_next, double* lam_prev,
+                          double* grad_force);
+hipError_t launch_loss(Ctx& c, const void* f, int kind, double* dev_value, double* dev_grad);
+hipError_t launch_axpy(Ctx& c, double a, const double* x, double* y, long long n);
+
+// Library context.
+struct Ctx {
+  Setup s;
+  KParams kp;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int precision = 0;
+  int kernel = 0;
+  size_t store_bytes = 8;     // bytes per stored f value
+  void* f[2] = {nullptr, nullptr};
+  int cur = 0;                // f[cur] holds the current state
+  void* fbx = nullptr;        // ghost-inclusive f-bar (two-pass kernels)
+  double* tab[3] = {nullptr, nullptr, nullptr};
+  double* wd[3] = {nullptr, nullptr, nullptr};
+  double* famb_dev = nullptr; // 3 doubles + controller state
+  double* fcell = nullptr;    // [3][N] additive centre force
+  double* red = nullptr;      // reduction scratch
+  size_t red_len = 0;
+  std::string err;
+  double bytes_allocated = 0;
+  // optional timing of the main step kernel (dugks_time_steps)
+  hipEvent_t tev[2] = {nullptr, nullptr};
+  bool timing = false;
+  std::vector<float> kernel_ms;
+  int launches_per_step = 0;
+
+  hipError_t alloc(void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) bytes_allocated += (double)bytes;
+    return e;
+  }
+};
+
+// Builds the kernel parameter block and the device tables for a setup.
+hipError_t build_params(Ctx& c);
+
+}  // namespace dugks

@@ -1,0 +1,167 @@
+// Restart dump and output writers (host C++).
+//   dump / read_dump     src/output.cu:22-89 (raw real values, f[k][j][i][ip])
+//   tecplot slices       src/output.cu:152-218
+//   diag.dat, profiles   src/output.cu:223-334
+#include "io.h"
+
+#include <cstdio>
+#include <fstream>
+#include <vector>
+
+#include "../../include/dugks.h"
+
+namespace dugks {
+
+std::string dump_name(int step) {
+  char buf[32];
+  std::snprintf(buf, sizeof buf, "ddf%8.8d.dat", step);
+  return buf;
+}
+
+std::string join_path(const std::string& dir, const std::string& name) {
+  if (dir.empty()) return name;
+  if (dir.back() == '/') return dir + name;
+  return dir + "/" + name;
+}
+
+void write_dump(const std::string& path, const double* f, size_t count, int real_bytes) {
+  if (real_bytes != 8 && real_bytes != 4) throw ConfigError(DUGKS_EARG, "dump real_bytes must be 4 or 8");
+  std::ofstream o(path, std::ios::binary);
+  if (!o) throw ConfigError(DUGKS_EIO, "cannot open '" + path + "' for writing");
+  if (real_bytes == 8) {
+    o.write(reinterpret_cast<const char*>(f), count * sizeof(double));
+  } else {
+    const size_t chunk = 1 << 20;
+    std::vector<float> buf;
+    for (size_t b = 0; b < count; b += chunk) {
+      size_t m = std::min(chunk, count - b);
+      buf.resize(m);
+      for (size_t q = 0; q < m; q++) buf[q] = (float)f[b + q];
+      o.write(reinterpret_cast<const char*>(buf.data()), m * sizeof(float));
+    }
+  }
+  if (!o) throw ConfigError(DUGKS_EIO, "write failed on '" + path + "'");
+}
+
+void read_dump(const std::string& path, double* f, size_t count, int real_bytes) {
+  if (real_bytes != 8 && real_bytes != 4) throw ConfigError(DUGKS_EARG, "dump real_bytes must be 4 or 8");
+  std::ifstream in(path, std::ios::binary | std::ios::ate);
+  if (!in) throw ConfigError(DUGKS_EIO, "cannot open dump '" + path + "'");
+  const std::streamoff size = in.tellg();
+  if ((size_t)size < count * (size_t)real_bytes)
+    throw ConfigError(DUGKS_EIO, "dump '" + path + "' is shorter than nx*ny*nz*19 values");
+  in.seekg(0);
+  if (real_bytes == 8) {
+    in.read(reinterpret_cast<char*>(f), count * sizeof(double));
+  } else {
+    const size_t chunk = 1 << 20;
+    std::vector<float> buf;
+    for (size_t b = 0; b < count; b += chunk) {
+      size_t m = std::min(chunk, count - b);
+      buf.resize(m);
+      in.read(reinterpret_cast<char*>(buf.data()), m * sizeof(float));
+      for (size_t q = 0; q < m; q++) f[b + q] = buf[q];
+    }
+  }
+  if (!in) throw ConfigError(DUGKS_EIO, "read failed on '" + path + "'");
+}
+
+namespace {
+
+void slice(const Setup& s, const char* tag, int fixed_axis, int step, const double* rho,
+           const double* ux, const double* uy, const double* uz, const std::string& dir) {
+  const int nx = s.n[0], ny = s.n[1], nz = s.n[2];
+  char name[32];
+  std::snprintf(name, sizeof name, "s2%s%8.8d.tec", tag, step);
+  std::ofstream file(join_path(dir, name));
+  if (!file) throw ConfigError(DUGKS_EIO, std::string("cannot write ") + name);
+  int dims[3] = {nx, ny, nz};
+  dims[fixed_axis] = 1;
+  file << "TITLE = \"Example: Simple 2D-Slice Data\"" << std::endl;
+  file << "VARIABLES = \"X\", \"Y\", \"Z\", \"RHO\", \"UX\", \"UY\", \"UZ\"" << std::endl;
+  file << "ZONE I=" << dims[0] << ", J=" << dims[1] << ", K=" << dims[2] << " F=POINT" << std::endl;
+  auto coord = [&](int a, int m) { return s.uniform[a] ? (m + 0.5) * s.du[a] : s.xc[a][m + 1]; };
+  const int fixed = s.n[fixed_axis] / 2;
+  for (int k = 0; k < (fixed_axis == 2 ? 1 : nz); k++)
+    for (int j = 0; j < (fixed_axis == 1 ? 1 : ny); j++)
+      for (int i = 0; i < (fixed_axis == 0 ? 1 : nx); i++) {
+        int idx[3] = {i, j, k};
+        idx[fixed_axis] = fixed;
+        size_t c = ((size_t)idx[2] * ny + idx[1]) * nx + idx[0];
+        file << coord(0, idx[0]) << " " << coord(1, idx[1]) << " " << coord(2, idx[2]) << " "
+             << rho[c] << " " << ux[c] << " " << uy[c] << " " << uz[c] << std::endl;
+      }
+}
+
+}  // namespace
+
+void write_outputs(const Setup& s, int step, const double* rho, const double* ux,
+                   const double* uy, const double* uz, const std::string& dir) {
+  slice(s, "X", 0, step, rho, ux, uy, uz, dir);
+  slice(s, "Y", 1, step, rho, ux, uy, uz, dir);
+  slice(s, "Z", 2, step, rho, ux, uy, uz, dir);
+  const int nx = s.n[0], ny = s.n[1], nz = s.n[2];
+  const size_t N = s.ncell();
+  double mean[4] = {0, 0, 0, 0}, mn[4], mx[4];
+  for (int q = 0; q < 4; q++) {
+    mn[q] = 99999;
+    mx[q] = -99999;
+  }
+  const double* F[4] = {rho, ux, uy, uz};
+  for (size_t c = 0; c < N; c++)
+    for (int q = 0; q < 4; q++) {
+      mean[q] += F[q][c];
+      if (F[q][c] < mn[q]) mn[q] = F[q][c];
+      if (F[q][c] > mx[q]) mx[q] = F[q][c];
+    }
+  for (int q = 0; q < 4; q++) mean[q] /= (double)N;
+  FILE* file = std::fopen(join_path(dir, "diag.dat").c_str(), "a");
+  if (!file) throw ConfigError(DUGKS_EIO, "cannot append diag.dat");
+  std::fprintf(file,
+               "%8.8d  % 7.7e % 7.7e % 7.7e % 7.7e  % 7.7e % 7.7e % 7.7e % 7.7e  % 7.7e % 7.7e "
+               "% 7.7e % 7.7e\n",
+               step, mean[0], mean[1], mean[2], mean[3], mn[0], mn[1], mn[2], mn[3], mx[0], mx[1],
+               mx[2], mx[3]);
+  std::fclose(file);
+
+  // per wall-normal index: means, Reynolds stresses xz/xy/yz, variances
+  int pa = s.wall_axis();
+  if (pa < 0) pa = 0;
+  const int np = s.n[pa];
+  std::vector<double> acc((size_t)np * 9, 0.0);
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        const int idx[3] = {i, j, k};
+        const size_t c = ((size_t)k * ny + j) * nx + i;
+        double* a = &acc[(size_t)idx[pa] * 9];
+        a[0] += ux[c];
+        a[1] += uy[c];
+        a[2] += uz[c];
+        a[3] += ux[c] * uz[c];
+        a[4] += ux[c] * uy[c];
+        a[5] += uy[c] * uz[c];
+        a[6] += ux[c] * ux[c];
+        a[7] += uy[c] * uy[c];
+        a[8] += uz[c] * uz[c];
+      }
+  const double cnt = (double)N / np;
+  file = std::fopen(join_path(dir, "profiles-2.dat").c_str(), "a");
+  if (!file) throw ConfigError(DUGKS_EIO, "cannot append profiles-2.dat");
+  std::fprintf(file, "%8.8d\n", step);
+  for (int m = 0; m < np; m++) {
+    double* a = &acc[(size_t)m * 9];
+    for (int q = 0; q < 9; q++) a[q] /= cnt;
+    a[6] -= a[0] * a[0];
+    a[7] -= a[1] * a[1];
+    a[8] -= a[2] * a[2];
+    a[3] -= a[0] * a[2];
+    a[4] -= a[0] * a[1];
+    a[5] -= a[1] * a[2];
+    std::fprintf(file, "% 7.7e % 7.7e % 7.7e   % 7.7e % 7.7e % 7.7e   % 7.7e % 7.7e % 7.7e \n", a[0],
+                 a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8]);
+  }
+  std::fclose(file);
+}
+
+}  // namespace dugks

@@ -1,0 +1,26 @@
+// Host file formats kept from the reference: the raw restart dump and the
+// Tecplot / diag.dat / profiles-2.dat outputs.
+#pragma once
+
+#include <string>
+
+#include "setup.h"
+
+namespace dugks {
+
+// "ddf%08d.dat" (src/output.cu:294-295)
+std::string dump_name(int step);
+std::string join_path(const std::string& dir, const std::string& name);
+
+// Raw dump of count doubles as FP64 (real_bytes = 8) or FP32 (4), no header
+// (src/output.cu:22-41); read is the inverse (src/output.cu:46-89).
+void write_dump(const std::string& path, const double* f, size_t count, int real_bytes);
+void read_dump(const std::string& path, double* f, size_t count, int real_bytes);
+
+// save_tecplot_serial (src/output.cu:152-218) and statistics_serial
+// (src/output.cu:223-334) with FP64 accumulators.  Profiles are taken along
+// the wall-normal axis (x in the reference), averaging over the other two.
+void write_outputs(const Setup& s, int step, const double* rho, const double* ux,
+                   const double* uy, const double* uz, const std::string& dir);
+
+}  // namespace dugks

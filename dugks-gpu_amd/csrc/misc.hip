@@ -1,0 +1,356 @@
+// Utility kernels: macroscopic output, state export/import, FP64 global
+// sums, constant-flow-rate controller, perturbation forcing, loss functionals.
+#include "device_common.h"
+
+namespace dugks {
+
+namespace {
+
+constexpr int kRedBlocks = 1024;  // fixed grid => fixed FP64 summation order
+constexpr int kRedThreads = 256;
+
+__device__ __forceinline__ void cell_coords(const KParams& P, long long c, int& i, int& j, int& k) {
+  i = (int)(c % P.n[0]);
+  const long long t = c / P.n[0];
+  j = (int)(t % P.n[1]);
+  k = (int)(t / P.n[1]);
+}
+
+__device__ __forceinline__ void ambient_d(const KParams& P, double F[3]) {
+  for (int a = 0; a < 3; a++) F[a] = P.famb_dev ? P.famb_dev[a] : P.famb[a];
+}
+
+// calc_macro (src/dugks.cu:1021-1064): rho = sum f, u = j/rho + h F_amb.
+template <class Store>
+__global__ void __launch_bounds__(256) k_macro(KParams P, const Store* __restrict__ f, double* rho,
+                                               double* ux, double* uy, double* uz) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= P.N) return;
+  double r = 0, u = 0, v = 0, w = 0;
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    const double x = Codec<Store>::template load<double>(f, ip * P.N + c, P, ip);
+    r += x;
+    u += CX[ip] * x;
+    v += CY[ip] * x;
+    w += CZ[ip] * x;
+  });
+  double F[3];
+  ambient_d(P, F);
+  rho[c] = r;
+  ux[c] = u / r + F[0] * P.hdt;
+  uy[c] = v / r + F[1] * P.hdt;
+  uz[c] = w / r + F[2] * P.hdt;
+}
+
+template <class Store>
+__global__ void __launch_bounds__(256) k_export(KParams P, const Store* __restrict__ f, double* dst) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P.N * Q) return;
+  const int ip = (int)(t / P.N);
+  dst[t] = Codec<Store>::template load<double>(f, t, P, ip);
+}
+
+template <class Store>
+__global__ void __launch_bounds__(256) k_import(KParams P, const double* __restrict__ src, Store* f) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P.N * Q) return;
+  const int ip = (int)(t / P.N);
+  Codec<Store>::template store<double>(f, t, src[t], P, ip);
+}
+
+// Fixed-order block reduction of NV doubles per thread into out[blockIdx*NV..].
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* out) {
+  __shared__ double sh[NV][kRedThreads];
+  for (int q = 0; q < NV; q++) sh[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int q = 0; q < NV; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < NV; q++) out[blockIdx.x * NV + q] = sh[q][0];
+}
+
+// Sums: mass, momentum (3), volume-weighted macro velocity (3), volume.
+template <class Store>
+__global__ void __launch_bounds__(kRedThreads) k_sums(KParams P, const Store* __restrict__ f,
+                                                      double* partial) {
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double F[3];
+  ambient_d(P, F);
+  for (long long c = (long long)blockIdx.x * kRedThreads + threadIdx.x; c < P.N;
+       c += (long long)kRedBlocks * kRedThreads) {
+    int i, j, k;
+    cell_coords(P, c, i, j, k);
+    const double dV = P.wd[0][i + 1] * P.wd[1][j + 1] * P.wd[2][k + 1];
+    double r = 0, u = 0, v = 0, w = 0;
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      const double x = Codec<Store>::template load<double>(f, ip * P.N + c, P, ip);
+      r += x;
+      u += CX[ip] * x;
+      v += CY[ip] * x;
+      w += CZ[ip] * x;
+    });
+    acc[0] += r;
+    acc[1] += u;
+    acc[2] += v;
+    acc[3] += w;
+    acc[4] += (u / r + F[0] * P.hdt) * dV;
+    acc[5] += (v / r + F[1] * P.hdt) * dV;
+    acc[6] += (w / r + F[2] * P.hdt) * dV;
+    acc[7] += dV;
+  }
+  block_reduce_store<8>(acc, partial);
+}
+
+template <int NV>
+__global__ void __launch_bounds__(kRedThreads) k_reduce_final(const double* partial, int nparts,
+                                                             double* out) {
+  double acc[NV];
+  for (int q = 0; q < NV; q++) acc[q] = 0;
+  for (int b = threadIdx.x; b < nparts; b += kRedThreads)
+    for (int q = 0; q < NV; q++) acc[q] += partial[b * NV + q];
+  __shared__ double sh[NV][kRedThreads];
+  for (int q = 0; q < NV; q++) sh[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int q = 0; q < NV; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < NV; q++) out[q] = sh[q][0];
+}
+
+// Constant-flow-rate controller (single thread).  famb_dev layout:
+// [0..2] ambient force, [3] previous bulk velocity, [4] previous-valid flag.
+// The wall drag is estimated from the last step's change of the bulk
+// velocity, U^n - U^{n-1} = dt (F^{n-1} - F_drag), and the new force closes
+// a fraction `gain` of the gap to the target in one step:
+// F^n = F_drag + gain (U_target - U^n) / dt.
+__global__ void k_flow_control(double* famb, const double* sums, int axis, double target, double gain,
+                               double dt) {
+  const double Ub = sums[4 + axis] / sums[7];
+  double Fprev = famb[axis];
+  double drag = Fprev;
+  if (famb[4] != 0) drag = Fprev - (Ub - famb[3]) / dt;
+  famb[axis] = drag + gain * (target - Ub) / dt;
+  famb[3] = Ub;
+  famb[4] = 1;
+}
+
+// calc_perturb_force (src/dugks.cu:1086-1126) written as the additive
+// centre force: streak forcing minus the ambient force inside the slab
+// 3 < i < nx/4 + 2 (1-based), zero elsewhere.
+__global__ void __launch_bounds__(256) k_perturb(KParams P, int step, double xwidth, double ywidth,
+                                                 double zwidth, double visc, double* fcell) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= P.N) return;
+  int i0, j0, k0;
+  cell_coords(P, c, i0, j0, k0);
+  const int i = i0 + 1, j = j0 + 1, k = k0 + 1, nx = P.n[0];
+  const double pi2 = 3.141592653589793 * 2;
+  const double beta = 3, Tpd = 2000, gamma = 2, Re_tau = 180;
+  const double ustar = 2 * Re_tau * visc / xwidth;
+  const double FF = 2 * pow(ustar, 2) / xwidth;
+  const double A0 = 5 * 40.0 / nx * beta * sin(pi2 * step * P.dt / Tpd);
+  double F[3];
+  ambient_d(P, F);
+  double fx = 0, fy = 0, fz = 0;
+  if (3 < i && i < nx / 4 + 2) {
+    const double xx = pi2 * (i - 3) * P.du[0] / (xwidth / 4.);
+    const double yy = pi2 * (j - 0.5) * P.du[1] / ywidth;
+    const double zz = pi2 * ((k - 0.5) * P.du[2]) / zwidth;
+    fx = 0.5 * FF * A0 * xwidth / 4 * (1 - cos(xx)) * cos(beta * yy) * cos(gamma * zz) - F[0];
+    fy = 0.5 * FF * A0 * ywidth / gamma * sin(xx) * cos(beta * yy) * sin(gamma * zz) - F[1];
+    fz = FF * (1 - A0 * zwidth / beta * sin(xx) * sin(beta * yy) * cos(gamma * zz)) - F[2];
+  }
+  fcell[c] = fx;
+  fcell[P.N + c] = fy;
+  fcell[2 * P.N + c] = fz;
+}
+
+// Loss functionals and their gradient with respect to f (SoA FP64).
+//   kind 0: mean wall shear stress of the wall-normal axis aw along the
+//           streamwise axis as: tau_w = nu rho u_s / delta at the first cell
+//           off each wall, rho u_s = sum_i (c_is + h F_s) f_i (linear in f);
+//           J = sum over both wall planes / (2 * cells per plane).
+//   kind 1: kinetic energy J = sum 0.5 rho |u|^2 dV / V, u = j/rho + h F.
+template <class Store>
+__global__ void __launch_bounds__(kRedThreads) k_loss(KParams P, const Store* __restrict__ f, int kind,
+                                                      int aw, int as, double visc, double vol,
+                                                      double nplane, double* grad, double* partial) {
+  double acc[1] = {0};
+  double F[3];
+  ambient_d(P, F);
+  for (long long c = (long long)blockIdx.x * kRedThreads + threadIdx.x; c < P.N;
+       c += (long long)kRedBlocks * kRedThreads) {
+    int idx[3];
+    cell_coords(P, c, idx[0], idx[1], idx[2]);
+    double x[Q];
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      x[ip] = Codec<Store>::template load<double>(f, ip * P.N + c, P, ip);
+    });
+    if (kind == 0) {
+      const int m = idx[aw];
+      const bool wall = m == 0 || m == P.n[aw] - 1;
+      double coef = 0;
+      if (wall) {
+        const double delta = 0.5 * P.wd[aw][m + 1];
+        coef = visc / (delta * 2.0 * nplane);
+      }
+      double val = 0;
+      static_for<Q>([&](auto I) {
+        constexpr int ip = decltype(I)::value;
+        const double g = coef * (cv(0, ip) * (as == 0) + cv(1, ip) * (as == 1) + cv(2, ip) * (as == 2) +
+                                 P.hdt * F[as]);
+        val += g * x[ip];
+        if (grad) grad[ip * P.N + c] = g;
+      });
+      acc[0] += val;
+    } else {
+      const double dV = P.wd[0][idx[0] + 1] * P.wd[1][idx[1] + 1] * P.wd[2][idx[2] + 1];
+      double r = 0, j[3] = {0, 0, 0};
+      static_for<Q>([&](auto I) {
+        constexpr int ip = decltype(I)::value;
+        r += x[ip];
+        j[0] += CX[ip] * x[ip];
+        j[1] += CY[ip] * x[ip];
+        j[2] += CZ[ip] * x[ip];
+      });
+      double u[3], uu = 0;
+      for (int a = 0; a < 3; a++) {
+        u[a] = j[a] / r + P.hdt * F[a];
+        uu += u[a] * u[a];
+      }
+      acc[0] += 0.5 * r * uu * dV / vol;
+      if (grad) {
+        static_for<Q>([&](auto I) {
+          constexpr int ip = decltype(I)::value;
+          const double cu = (CX[ip] + P.hdt * F[0]) * u[0] + (CY[ip] + P.hdt * F[1]) * u[1] +
+                            (CZ[ip] + P.hdt * F[2]) * u[2];
+          grad[ip * P.N + c] = (dV / vol) * (cu - 0.5 * uu);
+        });
+      }
+    }
+  }
+  block_reduce_store<1>(acc, partial);
+}
+
+__global__ void __launch_bounds__(256) k_axpy(double a, const double* __restrict__ x, double* y, long long n) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] += a * x[t];
+}
+
+template <class Store>
+hipError_t macro_t(Ctx& c, const void* f, double* rho, double* ux, double* uy, double* uz) {
+  const unsigned nb = (unsigned)((c.kp.N + 255) / 256);
+  hipLaunchKernelGGL((k_macro<Store>), dim3(nb), dim3(256), 0, c.stream, c.kp,
+                     static_cast<const Store*>(f), rho, ux, uy, uz);
+  return hipGetLastError();
+}
+
+template <class Store>
+hipError_t export_t(Ctx& c, const void* f, double* dst) {
+  const unsigned nb = (unsigned)((c.kp.N * Q + 255) / 256);
+  hipLaunchKernelGGL((k_export<Store>), dim3(nb), dim3(256), 0, c.stream, c.kp,
+                     static_cast<const Store*>(f), dst);
+  return hipGetLastError();
+}
+
+template <class Store>
+hipError_t import_t(Ctx& c, const double* src, void* f) {
+  const unsigned nb = (unsigned)((c.kp.N * Q + 255) / 256);
+  hipLaunchKernelGGL((k_import<Store>), dim3(nb), dim3(256), 0, c.stream, c.kp, src,
+                     static_cast<Store*>(f));
+  return hipGetLastError();
+}
+
+template <class Store>
+hipError_t sums_t(Ctx& c, const void* f, double* out) {
+  hipLaunchKernelGGL((k_sums<Store>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.stream, c.kp,
+                     static_cast<const Store*>(f), c.red);
+  hipLaunchKernelGGL((k_reduce_final<8>), dim3(1), dim3(kRedThreads), 0, c.stream, c.red, kRedBlocks,
+                     out);
+  return hipGetLastError();
+}
+
+template <class Store>
+hipError_t loss_t(Ctx& c, const void* f, int kind, double* value, double* grad) {
+  const Setup& s = c.s;
+  const int aw = s.wall_axis();
+  int as = s.stream_axis();
+  if (kind == 0 && aw < 0) return hipErrorInvalidValue;
+  if (kind == 0 && as == aw) as = (aw + 1) % 3;
+  double vol = s.width[0] * s.width[1] * s.width[2];
+  double nplane = aw >= 0 ? (double)s.ncell() / s.n[aw] : 1.0;
+  hipLaunchKernelGGL((k_loss<Store>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.stream, c.kp,
+                     static_cast<const Store*>(f), kind, aw < 0 ? 0 : aw, as, s.visc, vol, nplane, grad,
+                     c.red);
+  hipLaunchKernelGGL((k_reduce_final<1>), dim3(1), dim3(kRedThreads), 0, c.stream, c.red, kRedBlocks,
+                     value);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_macro(Ctx& c, const void* f, double* rho, double* ux, double* uy, double* uz) {
+  if (c.precision == 0) return macro_t<double>(c, f, rho, ux, uy, uz);
+  if (c.precision == 1) return macro_t<float>(c, f, rho, ux, uy, uz);
+  return macro_t<__half>(c, f, rho, ux, uy, uz);
+}
+
+hipError_t launch_export(Ctx& c, const void* f, double* dst) {
+  if (c.precision == 0) return export_t<double>(c, f, dst);
+  if (c.precision == 1) return export_t<float>(c, f, dst);
+  return export_t<__half>(c, f, dst);
+}
+
+hipError_t launch_import(Ctx& c, const double* src, void* f) {
+  if (c.precision == 0) return import_t<double>(c, src, f);
+  if (c.precision == 1) return import_t<float>(c, src, f);
+  return import_t<__half>(c, src, f);
+}
+
+hipError_t launch_global_sums(Ctx& c, const void* f, double* dev_out8) {
+  if (c.precision == 0) return sums_t<double>(c, f, dev_out8);
+  if (c.precision == 1) return sums_t<float>(c, f, dev_out8);
+  return sums_t<__half>(c, f, dev_out8);
+}
+
+hipError_t launch_flow_control(Ctx& c, const void* f) {
+  double* sums = c.red + kRedBlocks * 8;
+  hipError_t e = launch_global_sums(c, f, sums);
+  if (e != hipSuccess) return e;
+  int axis = c.s.stream_axis();
+  hipLaunchKernelGGL(k_flow_control, dim3(1), dim3(1), 0, c.stream, c.famb_dev, sums, axis,
+                     c.s.flow_rate, c.s.flow_gain, c.s.dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_perturb_force(Ctx& c, int step) {
+  const unsigned nb = (unsigned)((c.kp.N + 255) / 256);
+  hipLaunchKernelGGL(k_perturb, dim3(nb), dim3(256), 0, c.stream, c.kp, step, c.s.width[0], c.s.width[1],
+                     c.s.width[2], c.s.visc, c.fcell);
+  return hipGetLastError();
+}
+
+hipError_t launch_loss(Ctx& c, const void* f, int kind, double* dev_value, double* dev_grad) {
+  if (c.precision == 0) return loss_t<double>(c, f, kind, dev_value, dev_grad);
+  if (c.precision == 1) return loss_t<float>(c, f, kind, dev_value, dev_grad);
+  return loss_t<__half>(c, f, kind, dev_value, dev_grad);
+}
+
+hipError_t launch_axpy(Ctx& c, double a, const double* x, double* y, long long n) {
+  hipLaunchKernelGGL(k_axpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, a, x, y, n);
+  return hipGetLastError();
+}
+
+size_t reduction_scratch_doubles() { return (size_t)kRedBlocks * 8 + 16; }
+
+}  // namespace dugks
