@@ -59,7 +59,7 @@ def run_mode(dugks_amd, configs, torch, precision, args):
     res = dict(mlups=mlups, ms_per_step=wall * 1e3 / args.steps, kernel_ms=kt["mean_ms"],
                kernel_ms_min=kt["min_ms"], launches_per_step=int(kt["launches_per_step"]),
                alg_bytes_per_launch=alg_bytes, achieved_gbs=achieved,
-               hbm_frac=achieved / HBM_PEAK_GBS, kernel=int(eng.params["kernel"]), finite=ok)
+               hbm_frac=achieved / HBM_PEAK_GBS, launches=int(kt["launches_per_step"]), finite=bool(ok))
     log(f"{precision}: {mlups:.0f} MLUPS, kernel {kt['mean_ms']:.3f} ms, {achieved:.0f} GB/s algorithmic")
     eng.close()
     return res
@@ -137,7 +137,7 @@ def main():
         "config": {"workload": "Re_tau=180 channel 256x192x256, tanh-stretched y (alpha 0.85), "
                                "constant flow rate, log-law + Fourier initial state",
                    "grid": [256, 192, 256], "cells": 256 * 192 * 256, "precision": primary,
-                   "step_kernel": {0: "twopass", 1: "fused"}.get(r["kernel"], r["kernel"])},
+                   "step_kernel": "fused" if r["launches_per_step"] <= 2 else "twopass"},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None},

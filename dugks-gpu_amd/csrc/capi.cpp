@@ -160,7 +160,9 @@ int create(const Setup& s, const dugks_opts* opts, dugks_ctx** out) {
       dugks_free(holder.release());
       return rc;
     }
-    c.s.flow_rate = sums[4 + c.s.stream_axis()] / sums[7];
+    // target of the controller: the momentum bulk velocity <j/rho> of the
+    // initial state (the output velocity minus h F)
+    c.s.flow_rate = sums[4 + c.s.stream_axis()] / sums[7] - c.s.hdt * c.s.famb[c.s.stream_axis()];
   }
   if (o.verbose) {
     std::printf("dugks: grid %d x %d x %d, precision %d, kernel %d, dt=%.9g tau=%.9g\n", s.n[0], s.n[1],
@@ -194,7 +196,11 @@ int step_once(dugks_ctx* ctx, int istep) {
   void* fout = c.f[1 - c.cur];
   bool handled = false;
   if (c.kernel != DUGKS_KERNEL_TWOPASS) HIPCHK(launch_step_fused(c, fin, fout, &handled));
-  if (!handled) {
+  if (handled) {
+    c.cur = 1 - c.cur;  // the fused launch also ran the flow-rate controller
+    return DUGKS_OK;
+  }
+  {
     if (c.kernel == DUGKS_KERNEL_FUSED)
       return fail(ctx, DUGKS_EUNSUPPORTED, "the fused kernel does not cover this configuration");
     if (!c.fbx) {
@@ -246,6 +252,7 @@ hipError_t build_params(Ctx& c) {
   P.fw1 = s.fw1;
   P.fw2 = s.fw2;
   P.recon = s.recon;
+  P.flow_axis = s.stream_axis();
   for (int ip = 0; ip < Q; ip++) {
     P.hbase[ip] = (float)(wgt(ip) * s.rho0);
     P.hw[ip] = (float)wgt(ip);
@@ -258,6 +265,11 @@ hipError_t build_params(Ctx& c) {
     if ((e = hipMemcpy(c.tab[a], t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
       return e;
     P.tab[a] = c.tab[a];
+    std::vector<float> tf(t.begin(), t.end());
+    if ((e = c.alloc((void**)&c.tabf[a], tf.size() * sizeof(float))) != hipSuccess) return e;
+    if ((e = hipMemcpy(c.tabf[a], tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
+      return e;
+    P.tabf[a] = c.tabf[a];
     if ((e = c.alloc((void**)&c.wd[a], s.d[a].size() * sizeof(double))) != hipSuccess) return e;
     if ((e = hipMemcpy(c.wd[a], s.d[a].data(), s.d[a].size() * sizeof(double), hipMemcpyHostToDevice)) !=
         hipSuccess)
@@ -270,6 +282,10 @@ hipError_t build_params(Ctx& c) {
   P.famb_dev = c.famb_dev;
   c.red_len = reduction_scratch_doubles() + 64;
   if ((e = c.alloc((void**)&c.red, c.red_len * sizeof(double))) != hipSuccess) return e;
+  // per-block flow sums of the fused kernel: at most (nx/32)(ny/8) blocks
+  // per z-chunk, z-chunks of at least 1 plane
+  c.flow_partial_len = (size_t)(s.n[0] / 32 + 1) * (s.n[1] / 8 + 1) * (s.n[2] + 1);
+  if ((e = c.alloc((void**)&c.flow_partial, c.flow_partial_len * sizeof(double))) != hipSuccess) return e;
   return hipSuccess;
 }
 
@@ -357,11 +373,13 @@ void dugks_free(dugks_ctx* ctx) {
   hipFree(c.fbx);
   for (int a = 0; a < 3; a++) {
     hipFree(c.tab[a]);
+    hipFree(c.tabf[a]);
     hipFree(c.wd[a]);
   }
   hipFree(c.famb_dev);
   hipFree(c.fcell);
   hipFree(c.red);
+  hipFree(c.flow_partial);
   for (int q = 0; q < 2; q++)
     if (c.tev[q]) hipEventDestroy(c.tev[q]);
   if (c.own_stream && c.stream) hipStreamDestroy(c.stream);

@@ -61,9 +61,84 @@ struct Codec<__half> {
   }
 };
 
+// Raw buffer access to a state array: one buffer resource in SGPRs, a 32-bit
+// per-lane byte offset and a uniform byte offset per population, so 19
+// population loads share one address register.  Requires the array to span
+// fewer than 2^32 bytes; reads past the end return 0 and stores past it are
+// dropped by the hardware range check.
+template <class Store>
+struct StateBuf {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ StateBuf(const Store* p, unsigned bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<Store*>(p), 0, (int)bytes, 0x00020000)) {}
+  __device__ __forceinline__ Store raw(unsigned vb, unsigned sb) const;
+  __device__ __forceinline__ void put(Store v, unsigned vb, unsigned sb) const;
+};
+template <>
+__device__ __forceinline__ double StateBuf<double>::raw(unsigned vb, unsigned sb) const {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vb, (int)sb, 0));
+}
+template <>
+__device__ __forceinline__ float StateBuf<float>::raw(unsigned vb, unsigned sb) const {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vb, (int)sb, 0));
+}
+template <>
+__device__ __forceinline__ __half StateBuf<__half>::raw(unsigned vb, unsigned sb) const {
+  return __ushort_as_half(__builtin_amdgcn_raw_buffer_load_b16(r, (int)vb, (int)sb, 0));
+}
+template <>
+__device__ __forceinline__ void StateBuf<double>::put(double v, unsigned vb, unsigned sb) const {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)vb, (int)sb, 0);
+}
+template <>
+__device__ __forceinline__ void StateBuf<float>::put(float v, unsigned vb, unsigned sb) const {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)vb, (int)sb, 0);
+}
+template <>
+__device__ __forceinline__ void StateBuf<__half>::put(__half v, unsigned vb, unsigned sb) const {
+  __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(v), r, (int)vb, (int)sb, 0);
+}
+
+// Decode / encode one population through a StateBuf (the Codec rules).
+template <class Real, class Store>
+__device__ __forceinline__ Real ld_state(const StateBuf<Store>& b, unsigned vb, unsigned sb, const KParams& P,
+                                         int ip) {
+  if constexpr (std::is_same<Store, __half>::value)
+    return (Real)(P.hbase[ip] + P.hw[ip] * __half2float(b.raw(vb, sb)));
+  else
+    return (Real)b.raw(vb, sb);
+}
+template <class Real, class Store>
+__device__ __forceinline__ void st_state(const StateBuf<Store>& b, unsigned vb, unsigned sb, Real v,
+                                         const KParams& P, int ip) {
+  if constexpr (std::is_same<Store, __half>::value)
+    b.put(__float2half(((float)v - P.hbase[ip]) * P.hinvw[ip]), vb, sb);
+  else
+    b.put((Store)v, vb, sb);
+}
+
+// 1/x: correctly rounded in FP64; the hardware reciprocal (1 ulp) in FP32.
 template <class Real>
 __device__ __forceinline__ Real rcp_real(Real x) {
   return Real(1) / x;
+}
+template <>
+__device__ __forceinline__ float rcp_real<float>(float x) {
+  return __builtin_amdgcn_rcpf(x);
+}
+
+// Per-axis weight table entry in the arithmetic type (FP32 kernels read a
+// float copy of the FP64 tables).
+template <class Real>
+__device__ __forceinline__ Real tabv(const KParams& P, int a, int idx);
+template <>
+__device__ __forceinline__ double tabv<double>(const KParams& P, int a, int idx) {
+  return P.tab[a][idx];
+}
+template <>
+__device__ __forceinline__ float tabv<float>(const KParams& P, int a, int idx) {
+  return P.tabf[a][idx];
 }
 
 // Centre-stage f-bar+ of one cell (calc_fb1 of the reference, src/dugks.cu:1131-1196):

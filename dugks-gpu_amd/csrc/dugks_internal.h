@@ -37,11 +37,13 @@ struct KParams {
   const double* famb_dev;  // device ambient force (3 doubles), overrides famb when set
   const double* fcell;     // additive centre force [3][N], or nullptr
   const double* tab[3];
+  const float* tabf[3];    // float copies of tab for FP32 arithmetic
   const double* wd[3];     // ghost-inclusive cell widths, index 0..n+1
   double qlo[3], qhi[3];   // wall ghost = q*f(1) + (1-q)*f(2) (and mirrored at the top)
   int uniform[3];
   double du[3];
   int recon;
+  int flow_axis;           // streamwise axis of the flow-rate controller
   // FP16 deviation codec: f_i = base_i + w_i * s_i, s_i = (f_i - base_i) / w_i
   float hbase[Q];
   float hw[Q];
@@ -79,6 +81,7 @@ struct Ctx {
   void* fbx = nullptr;        // ghost-inclusive f-bar (two-pass kernels)
   double* tab[3] = {nullptr, nullptr, nullptr};
   double* wd[3] = {nullptr, nullptr, nullptr};
+  float* tabf[3] = {nullptr, nullptr, nullptr};
   double* famb_dev = nullptr; // 3 doubles + controller state
   double* fcell = nullptr;    // [3][N] additive centre force
   double* red = nullptr;      // reduction scratch
@@ -90,6 +93,10 @@ struct Ctx {
   bool timing = false;
   std::vector<float> kernel_ms;
   int launches_per_step = 0;
+  // fused kernel: z-chunk length and the per-block flow-rate partial sums
+  int fused_tz = 0;
+  double* flow_partial = nullptr;
+  size_t flow_partial_len = 0;
 
   hipError_t alloc(void** p, size_t bytes) {
     hipError_t e = hipMalloc(p, bytes);

@@ -128,19 +128,27 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce_final(const double* part
 
 // Constant-flow-rate controller (single thread).  famb_dev layout:
 // [0..2] ambient force, [3] previous bulk velocity, [4] previous-valid flag.
-// The wall drag is estimated from the last step's change of the bulk
-// velocity, U^n - U^{n-1} = dt (F^{n-1} - F_drag), and the new force closes
-// a fraction `gain` of the gap to the target in one step:
-// F^n = F_drag + gain (U_target - U^n) / dt.
-__global__ void k_flow_control(double* famb, const double* sums, int axis, double target, double gain,
-                               double dt) {
-  const double Ub = sums[4 + axis] / sums[7];
-  double Fprev = famb[axis];
+// The controlled quantity is the momentum bulk velocity U = <j/rho> (the
+// output velocity minus h F, which would respond to F instantly and make the
+// loop overshoot).  Per step U^{n+1} = U^n + dt (F^n - D) with D the wall
+// drag; D is estimated from the last step's change and the new force closes
+// a fraction `gain` of the gap: F^n = D + gain (U_target - U^n) / dt, so the
+// error obeys e^{n+1} = (1 - gain) e^n for a steady drag (stable 0 < gain < 2).
+__device__ __forceinline__ void flow_update(double* famb, int axis, double Ub, double target, double gain,
+                                            double dt) {
+  const double Fprev = famb[axis];
   double drag = Fprev;
   if (famb[4] != 0) drag = Fprev - (Ub - famb[3]) / dt;
   famb[axis] = drag + gain * (target - Ub) / dt;
   famb[3] = Ub;
   famb[4] = 1;
+}
+
+__global__ void k_flow_control(double* famb, const double* sums, int axis, double target, double gain,
+                               double dt, double hdt) {
+  // sums[4 + axis] / sums[7] is the bulk output velocity <j/rho + h F>
+  const double Ub = sums[4 + axis] / sums[7] - hdt * famb[axis];
+  flow_update(famb, axis, Ub, target, gain, dt);
 }
 
 // calc_perturb_force (src/dugks.cu:1086-1126) written as the additive
@@ -329,7 +337,7 @@ hipError_t launch_flow_control(Ctx& c, const void* f) {
   if (e != hipSuccess) return e;
   int axis = c.s.stream_axis();
   hipLaunchKernelGGL(k_flow_control, dim3(1), dim3(1), 0, c.stream, c.famb_dev, sums, axis,
-                     c.s.flow_rate, c.s.flow_gain, c.s.dt);
+                     c.s.flow_rate, c.s.flow_gain, c.s.dt, c.s.hdt);
   return hipGetLastError();
 }
 

@@ -1,11 +1,548 @@
-// Fused single-kernel DUGKS step (placeholder until the fused kernel lands).
+// Fused DUGKS step: one kernel per time step, f read once and written once.
+//
+// Decomposition.  A workgroup owns a TX x TY tile of (x, y) columns and
+// marches along z over a chunk of TZ planes.  Its threads are TY + 2 rows of
+// TX lanes (the interior rows and the two y-halo rows) plus one wave for the
+// two x-halo columns (corners included); every thread owns one column.  Each
+// plane it:
+//   - loads f of its column at plane p+1 and computes f-bar+ (calc_fb1 of
+//     the reference, src/dugks.cu:1131-1196); halo columns recompute it, wall
+//     ghost columns extrapolate it (src/dugks.cu:1234-1251 and the stretched
+//     form, src/fortran/channel/src/dugks.cuf:365-374);
+//   - reconstructs every face once (the reference computes each interior face
+//     twice, quirk Q5): the trilinear interpolation of src/dugks.cu:756-906 is
+//     evaluated z-lerp first, so the x- and y-faces of plane p read a single
+//     LDS plane Z(p) and the z-face between p and p+1 reads Zn(p+1/2);
+//   - applies wall bounce-back and the face collision (f2ruvwh + feq_s,
+//     src/dugks.cu:675-709, 1201-1207) and keeps only the populations that
+//     cross the face;
+//   - updates plane p (src/dugks.cu:945-956) into the other f buffer.
+// z must be periodic; x and y may be periodic or walls, uniform or stretched.
+// Other configurations use the two-pass kernels.
 #include "device_common.h"
 
 namespace dugks {
 
-hipError_t launch_step_fused(Ctx&, const void*, void*, bool* handled) {
+namespace {
+
+// populations with a non-zero component along each axis
+constexpr int XD[10] = {1, 2, 7, 8, 9, 10, 11, 12, 13, 14};
+constexpr int YD[10] = {3, 4, 7, 8, 9, 10, 15, 16, 17, 18};
+constexpr int ZD[10] = {5, 6, 11, 12, 13, 14, 15, 16, 17, 18};
+constexpr int ZP[5] = {5, 11, 12, 15, 16};  // c_z = +1
+
+template <int A>
+__device__ constexpr int dirs(int k) {
+  return A == 0 ? XD[k] : (A == 1 ? YD[k] : ZD[k]);
+}
+
+template <int TX, int TY>
+struct Tile {
+  static constexpr int NROW = (TY + 2) * TX;
+  static constexpr int NT = NROW + 64;
+  static constexpr int AC = TX + 2;
+  static_assert(2 * (TY + 2) <= 64, "x-halo columns must fit one wave");
+  static_assert(TX == 32 || TX == 64, "rows are half or whole waves");
+};
+
+template <class Real>
+__device__ __forceinline__ Real lerp_(Real a, Real b, Real m) {
+  return (Real(1) - m) * a + m * b;
+}
+
+template <class Real>
+__device__ __forceinline__ void ambient_force(const KParams& P, Real F[3]) {
+  F[0] = (Real)P.famb_dev[0];
+  F[1] = (Real)P.famb_dev[1];
+  F[2] = (Real)P.famb_dev[2];
+}
+
+// f-bar+ of interior cell (i, j, k) (0-based).
+template <class Real, class Store>
+__device__ __forceinline__ void fbar_cell(const KParams& P, const StateBuf<Store>& f, unsigned Nb, int i, int j,
+                                          int k, const Real Famb[3], Real (&fb)[Q]) {
+  const unsigned c = ((unsigned)k * P.n[1] + j) * P.n[0] + i;
+  Real ff[Q];
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    ff[ip] = ld_state<Real, Store>(f, c * (unsigned)sizeof(Store), ip * Nb, P, ip);
+  });
+  Real F[3] = {Famb[0], Famb[1], Famb[2]};
+  if (P.fcell) {
+    F[0] += (Real)P.fcell[c];
+    F[1] += (Real)P.fcell[P.N + c];
+    F[2] += (Real)P.fcell[2 * P.N + c];
+  }
+  centre_fbar<Real>(ff, F, (Real)P.hdt, (Real)P.bw0, (Real)P.bw1, (Real)P.bw2, fb);
+}
+
+// f-bar+ of a column that may be a wall ghost in x and/or y: the ghost value
+// is the linear extrapolation q f(1) + (1-q) f(2) from the wall side.
+// gi, gj are 0-based; a wall ghost is -1 or n; periodic indices are wrapped
+// by the caller.
+template <class Real, class Store>
+__device__ void fbar_column(const KParams& P, const StateBuf<Store>& f, unsigned Nb, int gi, int gj, int k,
+                            const Real Famb[3], Real (&fb)[Q]) {
+  const bool gx = gi < 0 || gi >= P.n[0];
+  const bool gy = gj < 0 || gj >= P.n[1];
+  if (!gx && !gy) {
+    fbar_cell<Real, Store>(P, f, Nb, gi, gj, k, Famb, fb);
+    return;
+  }
+  int xs[2] = {gi, gi}, ys[2] = {gj, gj};
+  Real wx[2] = {1, 0}, wy[2] = {1, 0};
+  if (gx) {
+    const bool lo = gi < 0;
+    const Real q = (Real)(lo ? P.qlo[0] : P.qhi[0]);
+    xs[0] = lo ? 0 : P.n[0] - 1;
+    xs[1] = lo ? 1 : P.n[0] - 2;
+    wx[0] = q;
+    wx[1] = Real(1) - q;
+  }
+  if (gy) {
+    const bool lo = gj < 0;
+    const Real q = (Real)(lo ? P.qlo[1] : P.qhi[1]);
+    ys[0] = lo ? 0 : P.n[1] - 1;
+    ys[1] = lo ? 1 : P.n[1] - 2;
+    wy[0] = q;
+    wy[1] = Real(1) - q;
+  }
+  // tensor-product extrapolation (the x -> y ghost fill of the reference
+  // gives the same linear combination)
+  bool first = true;
+  for (int b = 0; b < (gy ? 2 : 1); b++)
+    for (int a = 0; a < (gx ? 2 : 1); a++) {
+      const Real w = wx[a] * wy[b];
+      Real t[Q];
+      fbar_cell<Real, Store>(P, f, Nb, xs[a], ys[b], k, Famb, t);
+      static_for<Q>([&](auto I) {
+        constexpr int ip = decltype(I)::value;
+        fb[ip] = first ? w * t[ip] : fb[ip] + w * t[ip];
+      });
+      first = false;
+    }
+}
+
+// Index of population ip in the crossing set of axis A (-1 if it does not
+// cross a face normal to A).
+template <int A>
+__device__ constexpr int cross_index(int ip) {
+  for (int k = 0; k < 10; k++)
+    if (dirs<A>(k) == ip) return k;
+  return -1;
+}
+
+// Reconstructed populations of one face normal to axis A, held as the ten
+// values that cross the face plus the partial moments of the nine that do
+// not (those only enter rho and j of the face collision).
+template <class Real, int A>
+struct Face {
+  Real cross[10];
+  Real rho = 0, jx = 0, jy = 0, jz = 0;
+  template <int ip>
+  __device__ __forceinline__ void put(Real v) {
+    constexpr int k = cross_index<A>(ip);
+    if constexpr (k >= 0) {
+      cross[k] = v;
+    } else {
+      rho += v;
+      if constexpr (CX[ip] != 0) jx += CX[ip] > 0 ? v : -v;
+      if constexpr (CY[ip] != 0) jy += CY[ip] > 0 ? v : -v;
+      if constexpr (CZ[ip] != 0) jz += CZ[ip] > 0 ? v : -v;
+    }
+  }
+  // Half-way bounce-back: populations entering the domain through a wall
+  // take the value of their opposite (src/dugks.cu:807-823).
+  __device__ __forceinline__ void bounce(bool low_wall, bool high_wall) {
+    if (low_wall) {
+      static_for<10>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        constexpr int ip = dirs<A>(k);
+        if constexpr (cv(A, ip) > 0) cross[k] = cross[cross_index<A>(OPP[ip])];
+      });
+    }
+    if (high_wall) {
+      static_for<10>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        constexpr int ip = dirs<A>(k);
+        if constexpr (cv(A, ip) < 0) cross[k] = cross[cross_index<A>(OPP[ip])];
+      });
+    }
+  }
+  // Face collision (f2ruvwh + feq_s, src/dugks.cu:675-709, 1201-1207):
+  // rho = sum fbar, u = j/rho + (h/2) F, out = fw1 fbar + (fw2 + fw0 (c-u).F/RT) feq
+  // for the crossing populations.
+  __device__ __forceinline__ void collide(const Real F[3], Real hdt, Real fw0, Real fw1, Real fw2,
+                                          Real (&out)[10]) {
+    static_for<10>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      constexpr int ip = dirs<A>(k);
+      const Real v = cross[k];
+      rho += v;
+      if constexpr (CX[ip] != 0) jx += CX[ip] > 0 ? v : -v;
+      if constexpr (CY[ip] != 0) jy += CY[ip] > 0 ? v : -v;
+      if constexpr (CZ[ip] != 0) jz += CZ[ip] > 0 ? v : -v;
+    });
+    const Real inv = rcp_real(rho);
+    const Real hh = Real(0.5) * hdt;
+    const Real ux = jx * inv + hh * F[0];
+    const Real uy = jy * inv + hh * F[1];
+    const Real uz = jz * inv + hh * F[2];
+    const Real uu = ux * ux + uy * uy + uz * uz;
+    const Real base = Real(1) - Real(1.5) * uu;
+    const Real uF = ux * F[0] + uy * F[1] + uz * F[2];
+    const Real P0 = fw2 - Real(3) * fw0 * uF;
+    static_for<10>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      constexpr int ip = dirs<A>(k);
+      const Real cu = Real(CX[ip]) * ux + Real(CY[ip]) * uy + Real(CZ[ip]) * uz;
+      const Real cF = Real(CX[ip]) * F[0] + Real(CY[ip]) * F[1] + Real(CZ[ip]) * F[2];
+      const Real feq = Real(wgt(ip)) * rho * (base + Real(3) * cu + Real(4.5) * cu * cu);
+      out[k] = fw1 * cross[k] + (P0 + Real(3) * fw0 * cF) * feq;
+    });
+  }
+};
+
+template <class Real, class Store, int TX, int TY>
+__global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const Store* __restrict__ fin,
+                                                          Store* __restrict__ fout, int TZ,
+                                                          double* __restrict__ partial) {
+  using T = Tile<TX, TY>;
+  constexpr int AC = T::AC;
+  __shared__ Real A[Q][TY + 2][AC];        // Z(p), then Zn(p+1/2)
+  __shared__ Real Cx[10][TY][TX + 1];      // crossing populations of -x faces
+  __shared__ Real Dy[10][TY + 1][TX];      // crossing populations of -y faces
+
+  const int nx = P.n[0], ny = P.n[1], nz = P.n[2];
+  const int t = threadIdx.x;
+  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+  const int z0 = blockIdx.z * TZ;
+  const int zend = min(z0 + TZ, nz);
+  const unsigned Nb = (unsigned)(P.N * (long long)sizeof(Store));  // bytes per population
+  const StateBuf<Store> bin(fin, Q * Nb), bout(fout, Q * Nb);
+
+  // column of this thread in tile coordinates: r = 0..TY+1 (y = j0-1+r),
+  // c = -1..TX (x = i0+c)
+  int r, c;
+  bool active = true;
+  if (t < T::NROW) {
+    const int slot = t / TX;
+    c = t % TX;
+    r = slot == 0 ? 0 : (slot == 1 ? TY + 1 : slot - 1);
+  } else {
+    const int u = t - T::NROW;
+    active = u < 2 * (TY + 2);
+    const int side = u / (TY + 2);
+    r = u % (TY + 2);
+    c = side ? TX : -1;
+  }
+  const bool interior = t < T::NROW && r >= 1 && r <= TY;
+  int gi = i0 + c, gj = j0 - 1 + r;
+  if (P.periodic[0]) gi = (gi + nx) % nx;
+  if (P.periodic[1]) gj = (gj + ny) % ny;
+  // ghost-inclusive table indices of the column
+  const int mi = gi + 1, mj = gj + 1;
+  // weight tables in the arithmetic type (layout documented on KParams)
+  auto tx = [&](int idx) { return tabv<Real>(P, 0, idx); };
+  auto ty = [&](int idx) { return tabv<Real>(P, 1, idx); };
+  auto tz = [&](int idx) { return tabv<Real>(P, 2, idx); };
+
+  Real Famb[3];
+  ambient_force<Real>(P, Famb);
+  const Real hdt = (Real)P.hdt, fw0 = (Real)P.fw0, fw1 = (Real)P.fw1, fw2 = (Real)P.fw2;
+  // uniform z weights (z is periodic and uniform here)
+  const Real wzt_m = tz(9 + 6);  // tangential, toward p+1 (c_z = -1)
+  const Real wzt_p = tz(9 + 7);  // tangential, toward p-1 (c_z = +1)
+  const Real wzn[3] = {tz(9 + 0), tz(9 + 1), tz(9 + 2)};  // normal, toward p+1
+  const Real updz = tz(9 + 8);
+
+  Real fbm[5], fbc[Q], fbn[Q], zm[10];
+  double flow = 0;
+  const int as = P.flow_axis;
+
+  // Zn(p+1/2) of this column into A: z-normal lerp between fb(p) and fb(p+1)
+  auto store_zn = [&](const Real (&lo)[Q], const Real (&hi)[Q]) {
+    if (!active) return;
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      if constexpr (ip == 0)
+        A[0][r][c + 1] = Real(0.5) * (lo[0] + hi[0]);
+      else
+        A[ip][r][c + 1] = lerp_(lo[ip], hi[ip], wzn[CZ[ip] + 1]);
+    });
+  };
+  // +z face of an interior column from Zn in A: bilinear over the upwind
+  // tangential neighbours, then collision; out = crossing populations
+  auto zface = [&](Real (&out)[10]) {
+    Face<Real, 2> fc;
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      constexpr int cx = CX[ip], cy = CY[ip];
+      Real v = A[ip][r][c + 1];
+      if constexpr (cx != 0) {
+        const Real w = tx(mi * 9 + (cx < 0 ? 6 : 7));
+        v = lerp_(v, A[ip][r][c + 1 - cx], w);
+        if constexpr (cy != 0) {
+          const Real wy = ty(mj * 9 + (cy < 0 ? 6 : 7));
+          const Real v2 = lerp_(A[ip][r - cy][c + 1], A[ip][r - cy][c + 1 - cx], w);
+          v = lerp_(v, v2, wy);
+        }
+      } else if constexpr (cy != 0) {
+        const Real wy = ty(mj * 9 + (cy < 0 ? 6 : 7));
+        v = lerp_(v, A[ip][r - cy][c + 1], wy);
+      }
+      fc.template put<ip>(v);
+    });
+    fc.collide(Famb, hdt, fw0, fw1, fw2, out);
+  };
+
+  // ---- pre-loop: f-bar at z0-1 and z0, and the -z face of plane z0 ----
+  {
+    const int km = (z0 - 1 + nz) % nz;
+    if (active) {
+      fbar_column<Real, Store>(P, bin, Nb, gi, gj, km, Famb, fbn);
+      fbar_column<Real, Store>(P, bin, Nb, gi, gj, z0, Famb, fbc);
+      static_for<5>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        fbm[k] = fbn[ZP[k]];
+      });
+    }
+    store_zn(fbn, fbc);
+    __syncthreads();
+    if (interior) zface(zm);
+    __syncthreads();
+  }
+
+  for (int p = z0; p < zend; p++) {
+    const int kn = (p + 1) % nz;
+    if (active) fbar_column<Real, Store>(P, bin, Nb, gi, gj, kn, Famb, fbn);
+    // ---- phase 1: Z(p) = tangential z-lerp of f-bar(p) ----
+    if (active) {
+      static_for<Q>([&](auto I) {
+        constexpr int ip = decltype(I)::value;
+        constexpr int cz = CZ[ip];
+        if constexpr (cz == 0) {
+          A[ip][r][c + 1] = fbc[ip];
+        } else if constexpr (cz > 0) {
+          constexpr int zk = ip == 5 ? 0 : (ip == 11 ? 1 : (ip == 12 ? 2 : (ip == 15 ? 3 : 4)));
+          A[ip][r][c + 1] = lerp_(fbc[ip], fbm[zk], wzt_p);
+        } else {
+          A[ip][r][c + 1] = lerp_(fbc[ip], fbn[ip], wzt_m);
+        }
+      });
+    }
+    __syncthreads();
+    // ---- phase 2a: -x and -y faces of plane p ----
+    const bool do_x = interior || (active && t >= T::NROW && c == TX && r >= 1 && r <= TY);
+    const bool do_y = (t < T::NROW && r >= 1);  // interior rows and the top halo row
+    if (do_x) {
+      // face between x-columns gi-1 (left, ghost-inclusive index mi-1) and gi
+      Face<Real, 0> fc;
+      const int tl = (mi - 1) * 9;
+      static_for<Q>([&](auto I) {
+        constexpr int ip = decltype(I)::value;
+        constexpr int cx = CX[ip], cy = CY[ip];
+        if constexpr (ip == 0) {
+          fc.template put<0>(Real(0.5) * (A[0][r][c] + A[0][r][c + 1]));
+        } else {
+          const Real w = tx(tl + cx + 1);  // fraction toward the right cell
+          Real v = lerp_(A[ip][r][c], A[ip][r][c + 1], w);
+          if constexpr (cy != 0) {
+            const Real wy = ty(mj * 9 + (cy < 0 ? 6 : 7));
+            const Real v2 = lerp_(A[ip][r - cy][c], A[ip][r - cy][c + 1], w);
+            v = lerp_(v, v2, wy);
+          }
+          fc.template put<ip>(v);
+        }
+      });
+      if (!P.periodic[0]) fc.bounce(gi == 0, gi == nx);
+      Real out[10];
+      fc.collide(Famb, hdt, fw0, fw1, fw2, out);
+      static_for<10>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        Cx[k][r - 1][c] = out[k];
+      });
+    }
+    if (do_y) {
+      // face between y-rows gj-1 (below) and gj
+      Face<Real, 1> fc;
+      const int tb = (mj - 1) * 9;
+      static_for<Q>([&](auto I) {
+        constexpr int ip = decltype(I)::value;
+        constexpr int cx = CX[ip], cy = CY[ip];
+        if constexpr (ip == 0) {
+          fc.template put<0>(Real(0.5) * (A[0][r - 1][c + 1] + A[0][r][c + 1]));
+        } else {
+          const Real w = ty(tb + cy + 1);  // fraction toward the upper cell
+          Real lo = A[ip][r - 1][c + 1], hi = A[ip][r][c + 1];
+          if constexpr (cx != 0) {
+            const Real wx = tx(mi * 9 + (cx < 0 ? 6 : 7));
+            lo = lerp_(lo, A[ip][r - 1][c + 1 - cx], wx);
+            hi = lerp_(hi, A[ip][r][c + 1 - cx], wx);
+          }
+          fc.template put<ip>(lerp_(lo, hi, w));
+        }
+      });
+      if (!P.periodic[1]) fc.bounce(gj == 0, gj == ny);
+      Real out[10];
+      fc.collide(Famb, hdt, fw0, fw1, fw2, out);
+      static_for<10>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        Dy[k][r - 1][c] = out[k];
+      });
+    }
+    __syncthreads();
+    // ---- phase 2b: Zn(p+1/2) ----
+    store_zn(fbc, fbn);
+    __syncthreads();
+    // ---- phase 2c: +z face and update of plane p ----
+    if (interior) {
+      Real zp[10];
+      zface(zp);
+      const unsigned cell = ((unsigned)p * ny + gj) * nx + gi;
+      const unsigned vb = cell * (unsigned)sizeof(Store);
+      const Real updx = tx(mi * 9 + 8), updy = ty(mj * 9 + 8);
+      __builtin_amdgcn_sched_barrier(0);
+      // f~(p+1) = 4/3 f-bar+ - 1/3 f~ - sum_a (dt/dx_a) c_a (F_a+ - F_a-), one
+      // population at a time (fenced so the f reloads are not all hoisted)
+      Real fnew[Q];
+      static_for<Q>([&](auto I) {
+        constexpr int ip = decltype(I)::value;
+        const Real fo = ld_state<Real, Store>(bin, vb, ip * Nb, P, ip);
+        Real v = fbc[ip] * Real(4.0 / 3.0) - fo * Real(1.0 / 3.0);
+        constexpr int kx = cross_index<0>(ip), ky = cross_index<1>(ip), kz = cross_index<2>(ip);
+        if constexpr (kx >= 0) v -= Real(CX[ip]) * updx * (Cx[kx][r - 1][c + 1] - Cx[kx][r - 1][c]);
+        if constexpr (ky >= 0) v -= Real(CY[ip]) * updy * (Dy[ky][r][c] - Dy[ky][r - 1][c]);
+        if constexpr (kz >= 0) {
+          v -= Real(CZ[ip]) * updz * (zp[kz] - zm[kz]);
+          zm[kz] = zp[kz];
+        }
+        st_state<Real, Store>(bout, vb, ip * Nb, v, P, ip);
+        fnew[ip] = v;
+        if constexpr (ip % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+      });
+      if (partial) {
+        // bulk velocity of the new state for the flow-rate controller
+        double rr = 0, js = 0;
+        static_for<Q>([&](auto I) {
+          constexpr int ip = decltype(I)::value;
+          rr += (double)fnew[ip];
+          js += (double)cv(0, ip) * (as == 0) * (double)fnew[ip] + (double)cv(1, ip) * (as == 1) * (double)fnew[ip] +
+                (double)cv(2, ip) * (as == 2) * (double)fnew[ip];
+        });
+        const double dV = P.wd[0][mi] * P.wd[1][mj] * P.wd[2][p + 1];
+        flow += js / rr * dV;  // momentum bulk velocity, see k_flow_control
+      }
+    }
+    if (active) {
+      static_for<5>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        fbm[k] = fbc[ZP[k]];
+      });
+      static_for<Q>([&](auto I) {
+        constexpr int ip = decltype(I)::value;
+        fbc[ip] = fbn[ip];
+      });
+    }
+    __syncthreads();
+  }
+
+  if (partial) {
+    // fixed-order block reduction of the flow sums
+    double* red = reinterpret_cast<double*>(&A[0][0][0]);
+    red[t] = flow;
+    __syncthreads();
+    for (int s = 1; s < T::NT; s <<= 1) {
+      const int s2 = s << 1;
+      if ((t % s2) == 0 && t + s < T::NT) red[t] += red[t + s];
+      __syncthreads();
+    }
+    if (t == 0)
+      partial[((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+  }
+}
+
+// Final reduction of the per-block flow sums <j/rho> dV and the flow-rate
+// controller update (the law of k_flow_control).
+__global__ void __launch_bounds__(256) k_flow_finish(const double* partial, int nparts, double* famb, int axis,
+                                                     double volume, double target, double gain, double dt) {
+  __shared__ double sh[256];
+  double s = 0;
+  for (int b = threadIdx.x; b < nparts; b += 256) s += partial[b];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double Ub = sh[0] / volume;
+    const double Fprev = famb[axis];
+    double drag = Fprev;
+    if (famb[4] != 0) drag = Fprev - (Ub - famb[3]) / dt;
+    famb[axis] = drag + gain * (target - Ub) / dt;
+    famb[3] = Ub;
+    famb[4] = 1;
+  }
+}
+
+template <class Real, class Store, int TX, int TY>
+hipError_t run_fused(Ctx& cx, const void* fin, void* fout) {
+  const KParams& P = cx.kp;
+  using T = Tile<TX, TY>;
+  const int TZ = cx.fused_tz > 0 ? cx.fused_tz : 32;
+  dim3 grid(P.n[0] / TX, P.n[1] / TY, (P.n[2] + TZ - 1) / TZ);
+  const long long nblocks = (long long)grid.x * grid.y * grid.z;
+  double* partial = nullptr;
+  const bool flow = cx.s.flow_rate != 0;
+  if (flow) {
+    if ((size_t)nblocks > cx.flow_partial_len) return hipErrorInvalidValue;
+    partial = cx.flow_partial;
+  }
+  if (cx.timing) hipEventRecord(cx.tev[0], cx.stream);
+  hipLaunchKernelGGL((k_fused<Real, Store, TX, TY>), grid, dim3(T::NT), 0, cx.stream, P,
+                     static_cast<const Store*>(fin), static_cast<Store*>(fout), TZ, partial);
+  if (cx.timing) hipEventRecord(cx.tev[1], cx.stream);
+  if (flow) {
+    const Setup& s = cx.s;
+    const double vol = s.width[0] * s.width[1] * s.width[2];
+    hipLaunchKernelGGL(k_flow_finish, dim3(1), dim3(256), 0, cx.stream, partial, (int)nblocks, cx.famb_dev,
+                       s.stream_axis(), vol, s.flow_rate, s.flow_gain, s.dt);
+  }
+  cx.launches_per_step = flow ? 2 : 1;
+  return hipGetLastError();
+}
+
+}  // namespace
+
+bool fused_supported(const Ctx& c, int* tx, int* ty) {
+  const Setup& s = c.s;
+  if (s.recon != 0 || !s.periodic[2] || !s.uniform[2]) return false;
+  // buffer offsets are 32-bit: one state array must span < 4 GiB
+  if ((double)s.ncell() * Q * c.store_bytes >= 4294967296.0) return false;
+  *tx = (c.precision != 0 && s.n[0] % 64 == 0) ? 64 : 32;
+  *ty = 8;
+  if (s.n[0] % *tx != 0 || s.n[1] % *ty != 0) return false;
+  // a wall axis needs two interior cells per side of the extrapolation
+  if (!s.periodic[0] && s.n[0] < 3) return false;
+  if (!s.periodic[1] && s.n[1] < 3) return false;
+  return true;
+}
+
+hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled) {
+  int tx = 32, ty = 8;
   *handled = false;
-  return hipSuccess;
+  if (!fused_supported(c, &tx, &ty)) return hipSuccess;
+  *handled = true;
+  const bool wide = tx == 64;
+  switch (c.precision) {
+    case 0:
+      return run_fused<double, double, 32, 8>(c, fin, fout);
+    case 1:
+      return wide ? run_fused<float, float, 64, 8>(c, fin, fout) : run_fused<float, float, 32, 8>(c, fin, fout);
+    default:
+      return wide ? run_fused<float, __half, 64, 8>(c, fin, fout) : run_fused<float, __half, 32, 8>(c, fin, fout);
+  }
 }
 
 }  // namespace dugks

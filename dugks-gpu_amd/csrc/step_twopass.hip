@@ -213,7 +213,7 @@ __global__ void __launch_bounds__(256) k_flux(KParams P, const Store* __restrict
   static_for<Q>([&](auto I) {
     constexpr int ip = decltype(I)::value;
     const Real fo = Codec<Store>::template load<Real>(fin, ip * P.N + c, P, ip);
-    acc[ip] = fbx[ip * P.eN + e] * Real(4) / Real(3) - fo * Real(1) / Real(3);
+    acc[ip] = fbx[ip * P.eN + e] * Real(4.0 / 3.0) - fo * Real(1.0 / 3.0);
   });
   auto face = [&](auto Ac, auto Sc) {
     constexpr int A = decltype(Ac)::value;

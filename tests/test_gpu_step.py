@@ -18,7 +18,7 @@ from pyoracle import Oracle
 
 pytestmark = pytest.mark.gpu
 
-KERNELS = [KERNEL_TWOPASS]
+KERNELS = [KERNEL_TWOPASS, KERNEL_FUSED]
 
 
 def _engine(text, precision=PREC_FP64, kernel=KERNEL_TWOPASS):
@@ -43,25 +43,32 @@ def _relerr(a, b):
     return float(np.abs(a - b).max() / np.abs(b).max())
 
 
+# cases both step kernels cover (the fused kernel needs nx % 32 == 0,
+# ny % 8 == 0 and a periodic uniform z axis)
 CASES = {
-    "periodic_cube": dict(nx=8, ny=8, nz=8),
-    "periodic_ragged": dict(nx=13, ny=6, nz=5, xwidth=1.3, ywidth=0.6, zwidth=0.5),
-    "walls_x": dict(nx=9, ny=6, nz=4, periods_x=0, ambient_fz=2e-4, xwidth=0.9, ywidth=0.6, zwidth=0.4),
-    "walls_y": dict(nx=6, ny=10, nz=4, periods_y=0, ambient_fx=2e-4, xwidth=0.6, ywidth=1.0, zwidth=0.4),
+    "periodic": dict(nx=32, ny=8, nz=4, xwidth=3.2, ywidth=0.8, zwidth=0.4),
+    "periodic_wide": dict(nx=64, ny=8, nz=3, xwidth=6.4, ywidth=0.8, zwidth=0.3),
+    "walls_x": dict(nx=32, ny=8, nz=4, periods_x=0, ambient_fz=2e-4, xwidth=3.2, ywidth=0.8, zwidth=0.4),
+    "walls_y": dict(nx=32, ny=16, nz=4, periods_y=0, ambient_fx=2e-4, xwidth=3.2, ywidth=1.6, zwidth=0.4),
+    "walls_xy": dict(nx=32, ny=16, nz=3, periods_x=0, periods_y=0, ambient_fz=1e-4, xwidth=3.2,
+                     ywidth=1.6, zwidth=0.3),
+    "stretched_y": dict(nx=32, ny=16, nz=4, periods_y=0, mesh_y="tanh:0.85", dt_rule="min",
+                        ambient_fx=2e-4, xwidth=3.2, ywidth=2.0, zwidth=0.4),
+    "stretched_x": dict(nx=32, ny=8, nz=4, periods_x=0, mesh_x="tanh:0.8", dt_rule="min",
+                        ambient_fy=2e-4, xwidth=2.0, ywidth=0.8, zwidth=0.4),
+    "force_all": dict(nx=32, ny=8, nz=4, ambient_fx=1e-4, ambient_fy=-2e-4, ambient_fz=3e-4,
+                      xwidth=3.2, ywidth=0.8, zwidth=0.4),
+}
+# configurations only the two-pass kernels cover
+TWOPASS_CASES = {
+    "ragged": dict(nx=13, ny=6, nz=5, xwidth=1.3, ywidth=0.6, zwidth=0.5),
     "walls_z": dict(nx=6, ny=4, nz=10, periods_z=0, ambient_fy=2e-4, xwidth=0.6, ywidth=0.4, zwidth=1.0),
-    "walls_xy": dict(nx=8, ny=7, nz=4, periods_x=0, periods_y=0, ambient_fz=1e-4, xwidth=0.8,
-                     ywidth=0.7, zwidth=0.4),
-    "stretched_y": dict(nx=6, ny=16, nz=4, periods_y=0, mesh_y="tanh:0.85", dt_rule="min",
-                        ambient_fx=2e-4, xwidth=0.6, ywidth=2.0, zwidth=0.4),
     "grad": dict(nx=8, ny=6, nz=5, recon="grad", ambient_fx=1e-4, xwidth=0.8, ywidth=0.6, zwidth=0.5),
-    "force_all": dict(nx=6, ny=6, nz=6, ambient_fx=1e-4, ambient_fy=-2e-4, ambient_fz=3e-4),
 }
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
-@pytest.mark.parametrize("case", list(CASES))
-def test_fp64_one_step_matches_oracle(rng, case, kernel):
-    text = configs.base(visc=0.02, CFL=0.5, **CASES[case])
+def _one_step_parity(rng, case, kernel):
+    text = configs.base(visc=0.02, CFL=0.5, **case)
     eng = _engine(text, kernel=kernel)
     ora = _oracle_for(eng, text)
     f0 = random_state(eng.shape, rng)
@@ -76,10 +83,31 @@ def test_fp64_one_step_matches_oracle(rng, case, kernel):
     assert _relerr(eng.get_f(), ora.get_f()) <= 1e-12
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("kernel", KERNELS, ids=["twopass", "fused"])
+@pytest.mark.parametrize("case", list(CASES))
+def test_fp64_one_step_matches_oracle(rng, case, kernel):
+    _one_step_parity(rng, CASES[case], kernel)
+
+
+@pytest.mark.parametrize("case", list(TWOPASS_CASES))
+def test_fp64_one_step_matches_oracle_twopass_only(rng, case):
+    _one_step_parity(rng, TWOPASS_CASES[case], KERNEL_TWOPASS)
+    # the automatic choice falls back to the two-pass kernels here
+    eng = dugks_amd.Engine(configs.base(visc=0.02, **TWOPASS_CASES[case]))
+    assert eng.time_steps(1)["launches_per_step"] == 5
+    with pytest.raises(dugks_amd.DugksError):
+        _engine(configs.base(visc=0.02, **TWOPASS_CASES[case]), kernel=KERNEL_FUSED).step(1)
+
+
+def test_auto_selects_fused_kernel():
+    eng = dugks_amd.Engine(configs.base(visc=0.02, **CASES["walls_y"]))
+    assert eng.time_steps(2)["launches_per_step"] == 1
+
+
+@pytest.mark.parametrize("kernel", KERNELS, ids=["twopass", "fused"])
 def test_cell_force_matches_oracle(rng, kernel):
-    text = configs.base(nx=8, ny=6, nz=6, periods_y=0, visc=0.02, ambient_fx=2e-4,
-                        xwidth=0.8, ywidth=0.6, zwidth=0.6)
+    text = configs.base(nx=32, ny=8, nz=4, periods_y=0, visc=0.02, ambient_fx=2e-4,
+                        xwidth=3.2, ywidth=0.8, zwidth=0.4)
     eng = _engine(text, kernel=kernel)
     ora = _oracle_for(eng, text)
     f0 = random_state(eng.shape, rng)
@@ -93,11 +121,12 @@ def test_cell_force_matches_oracle(rng, kernel):
     assert _relerr(eng.get_f(), ora.get_f()) <= 1e-13
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
-@pytest.mark.parametrize("precision,tol", [(PREC_FP32, 2e-6), (PREC_FP16S, 1.5e-3)])
-def test_reduced_precision_one_step(rng, precision, tol, kernel):
-    text = configs.base(nx=16, ny=12, nz=8, periods_y=0, visc=0.02, ambient_fx=2e-4,
-                        xwidth=1.6, ywidth=1.2, zwidth=0.8)
+@pytest.mark.parametrize("kernel", KERNELS, ids=["twopass", "fused"])
+@pytest.mark.parametrize("nx", [32, 64])
+@pytest.mark.parametrize("precision,tol", [(PREC_FP32, 2e-6), (PREC_FP16S, 1.5e-3)], ids=["fp32", "fp16"])
+def test_reduced_precision_one_step(rng, precision, tol, kernel, nx):
+    text = configs.base(nx=nx, ny=16, nz=4, periods_y=0, visc=0.02, ambient_fx=2e-4,
+                        xwidth=nx / 10, ywidth=1.6, zwidth=0.4)
     eng = _engine(text, precision=precision, kernel=kernel)
     ora = _oracle_for(eng, text)
     f0 = random_state(eng.shape, rng)
@@ -108,9 +137,9 @@ def test_reduced_precision_one_step(rng, precision, tol, kernel):
     assert _relerr(eng.get_f(), ora.get_f()) <= tol
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("kernel", KERNELS, ids=["twopass", "fused"])
 def test_macro_and_sums_match_oracle(rng, kernel):
-    text = configs.base(nx=8, ny=8, nz=4, periods_x=0, visc=0.02, ambient_fy=3e-4, xwidth=0.8,
+    text = configs.base(nx=32, ny=8, nz=4, periods_x=0, visc=0.02, ambient_fy=3e-4, xwidth=0.8,
                         ywidth=0.8, zwidth=0.4)
     eng = _engine(text, kernel=kernel)
     ora = _oracle_for(eng, text)
@@ -124,12 +153,15 @@ def test_macro_and_sums_match_oracle(rng, kernel):
     assert abs(s[0] - rho.sum()) <= 1e-13 * rho.sum()
     assert np.allclose(s[1:4], j.sum((0, 1, 2)), rtol=1e-12, atol=1e-13)
     assert abs(s[7] - 0.8 * 0.8 * 0.4) < 1e-14
+    eng.step(1)  # exercises the selected kernel
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("kernel", KERNELS, ids=["twopass", "fused"])
 def test_laminar_channel_startup_against_series(kernel):
-    # 8 x 64 x 8, walls at y = +-1, Uc = 0.1, Re = 100, from rest
-    text = configs.laminar_channel(nsteps=20000)
+    # 8 x 64 x 8 (32 x 64 x 8 for the fused kernel), walls at y = +-1,
+    # Uc = 0.1, Re = 100, from rest, 20000 steps
+    nx = 8 if kernel == KERNEL_TWOPASS else 32
+    text = configs.laminar_channel(nsteps=20000, nx=nx, xwidth=nx * 2 / 64)
     eng = _engine(text, kernel=kernel)
     n = 20000
     eng.step(n)
@@ -148,9 +180,10 @@ def test_laminar_channel_startup_against_series(kernel):
     assert np.abs(uy).max() < 1e-12 and np.abs(uz).max() < 1e-12
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("kernel", KERNELS, ids=["twopass", "fused"])
 def test_laminar_channel_matches_oracle_over_2000_steps(kernel):
-    text = configs.laminar_channel()
+    nx = 8 if kernel == KERNEL_TWOPASS else 32
+    text = configs.laminar_channel(nx=nx, xwidth=nx * 2 / 64)
     eng = _engine(text, kernel=kernel)
     ora = _oracle_for(eng, text)
     ora.set_f(eng.get_f())
@@ -159,7 +192,7 @@ def test_laminar_channel_matches_oracle_over_2000_steps(kernel):
     assert _relerr(eng.get_f(), ora.get_f()) <= 1e-12
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("kernel", KERNELS, ids=["twopass", "fused"])
 def test_mass_and_momentum_input_at_scale(kernel):
     # 64^3 Taylor-Green with a uniform force: mass conserved, momentum grows
     # by dt * M * F per step (closed form of the scheme)
@@ -175,7 +208,7 @@ def test_mass_and_momentum_input_at_scale(kernel):
     assert np.allclose(dj, expect, rtol=1e-8)
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("kernel", KERNELS, ids=["twopass", "fused"])
 def test_restart_continues_bitwise(tmp_path, kernel):
     text = configs.turbulent_channel(nx=32, ny=24, nz=16, flow_rate=0.0)
     a = _engine(text, kernel=kernel)
@@ -202,15 +235,18 @@ def test_restart_from_config(tmp_path):
     assert (b.get_f() == a.get_f()).all()
 
 
-def test_flow_rate_control_holds_bulk_velocity():
+@pytest.mark.parametrize("kernel", KERNELS, ids=["twopass", "fused"])
+def test_flow_rate_control_holds_bulk_velocity(kernel):
     text = configs.turbulent_channel(nx=32, ny=32, nz=16)
-    eng = _engine(text)
+    eng = _engine(text, kernel=kernel)
+    h = eng.params["hdt"]
     s0 = eng.global_sums()
-    ub0 = s0[4] / s0[7]
+    F0 = eng.ambient_force()
+    ub0 = s0[4] / s0[7] - h * F0[0]  # momentum bulk velocity <j/rho>
     eng.step(300)
     s1 = eng.global_sums()
-    assert abs(s1[4] / s1[7] - ub0) < 1e-6 * ub0
     F = eng.ambient_force()
+    assert abs(s1[4] / s1[7] - h * F[0] - ub0) < 1e-6 * ub0
     assert F[0] > 0 and F[1] == 0 and F[2] == 0
 
 
