@@ -54,6 +54,7 @@ struct Ctx;
 
 // Launchers (implemented in the .hip files; all stream-ordered).
 hipError_t launch_step_twopass(Ctx& c, const void* fin, void* fout);
+hipError_t launch_fbar_ghosts(Ctx& c, const void* fin, void* fbx);
 hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled);
 hipError_t launch_macro(Ctx& c, const void* f, double* rho, double* ux, double* uy, double* uz);
 hipError_t launch_export(Ctx& c, const void* f, double* dst);
@@ -78,7 +79,8 @@ struct Ctx {
   size_t store_bytes = 8;     // bytes per stored f value
   void* f[2] = {nullptr, nullptr};
   int cur = 0;                // f[cur] holds the current state
-  void* fbx = nullptr;        // ghost-inclusive f-bar (two-pass kernels)
+  void* fbx = nullptr;        // ghost-inclusive f-bar (two-pass kernels, adjoint)
+  double* adj_faces = nullptr; // adjoint: 4 scalars per face, 3 axes, ghost-inclusive
   double* tab[3] = {nullptr, nullptr, nullptr};
   double* wd[3] = {nullptr, nullptr, nullptr};
   float* tabf[3] = {nullptr, nullptr, nullptr};

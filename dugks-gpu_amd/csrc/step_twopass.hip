@@ -275,7 +275,36 @@ hipError_t run_twopass(Ctx& cx, const void* fin, void* fout) {
   return hipGetLastError();
 }
 
+template <class Real, class Store>
+hipError_t run_fbar_ghosts(Ctx& cx, const void* fin, void* fbx_) {
+  const KParams& P = cx.kp;
+  const int bs = 256;
+  const unsigned nb = (unsigned)((P.N + bs - 1) / bs);
+  Real* fbx = static_cast<Real*>(fbx_);
+  hipLaunchKernelGGL((k_fbar<Real, Store>), dim3(nb), dim3(bs), 0, cx.stream, P,
+                     static_cast<const Store*>(fin), fbx);
+  for (int a = 0; a < 3; a++) {
+    const long long cnt = (long long)P.e[(a + 1) % 3] * P.e[(a + 2) % 3];
+    hipLaunchKernelGGL((k_ghost<Real>), dim3((unsigned)((cnt + bs - 1) / bs)), dim3(bs), 0, cx.stream,
+                       P, fbx, a);
+  }
+  return hipGetLastError();
+}
+
 }  // namespace
+
+// f-bar+ of every cell with the ghost layers filled (the first two stages of
+// the two-pass step), into a ghost-inclusive SoA array of the arithmetic type.
+hipError_t launch_fbar_ghosts(Ctx& c, const void* fin, void* fbx) {
+  switch (c.precision) {
+    case 0:
+      return run_fbar_ghosts<double, double>(c, fin, fbx);
+    case 1:
+      return run_fbar_ghosts<float, float>(c, fin, fbx);
+    default:
+      return run_fbar_ghosts<float, __half>(c, fin, fbx);
+  }
+}
 
 hipError_t launch_step_twopass(Ctx& c, const void* fin, void* fout) {
   switch (c.precision) {
