@@ -253,11 +253,7 @@ hipError_t build_params(Ctx& c) {
   P.fw2 = s.fw2;
   P.recon = s.recon;
   P.flow_axis = s.stream_axis();
-  for (int ip = 0; ip < Q; ip++) {
-    P.hbase[ip] = (float)(wgt(ip) * s.rho0);
-    P.hw[ip] = (float)wgt(ip);
-    P.hinvw[ip] = (float)(1.0 / wgt(ip));
-  }
+  P.rho0f = (float)s.rho0;
   hipError_t e;
   for (int a = 0; a < 3; a++) {
     std::vector<double> t = axis_table(s, a);

@@ -48,16 +48,16 @@ struct Codec<float> {
   }
 };
 
-// FP16 deviation storage: s_i = (f_i - w_i rho0) / w_i, f_i = w_i rho0 + w_i s_i.
+// FP16 deviation storage: s_i = (f_i - w_i rho0) / w_i, f_i = w_i (rho0 + s_i).
 template <>
 struct Codec<__half> {
   template <class Real>
   __device__ __forceinline__ static Real load(const __half* p, long long i, const KParams& P, int ip) {
-    return (Real)(P.hbase[ip] + P.hw[ip] * __half2float(p[i]));
+    return (Real)((float)wgt(ip) * (P.rho0f + __half2float(p[i])));
   }
   template <class Real>
   __device__ __forceinline__ static void store(__half* p, long long i, Real v, const KParams& P, int ip) {
-    p[i] = __float2half(((float)v - P.hbase[ip]) * P.hinvw[ip]);
+    p[i] = __float2half((float)v * (float)(1.0 / wgt(ip)) - P.rho0f);
   }
 };
 
@@ -105,7 +105,7 @@ template <class Real, class Store>
 __device__ __forceinline__ Real ld_state(const StateBuf<Store>& b, unsigned vb, unsigned sb, const KParams& P,
                                          int ip) {
   if constexpr (std::is_same<Store, __half>::value)
-    return (Real)(P.hbase[ip] + P.hw[ip] * __half2float(b.raw(vb, sb)));
+    return (Real)((float)wgt(ip) * (P.rho0f + __half2float(b.raw(vb, sb))));
   else
     return (Real)b.raw(vb, sb);
 }
@@ -113,7 +113,7 @@ template <class Real, class Store>
 __device__ __forceinline__ void st_state(const StateBuf<Store>& b, unsigned vb, unsigned sb, Real v,
                                          const KParams& P, int ip) {
   if constexpr (std::is_same<Store, __half>::value)
-    b.put(__float2half(((float)v - P.hbase[ip]) * P.hinvw[ip]), vb, sb);
+    b.put(__float2half((float)v * (float)(1.0 / wgt(ip)) - P.rho0f), vb, sb);
   else
     b.put((Store)v, vb, sb);
 }

@@ -44,10 +44,8 @@ struct KParams {
   double du[3];
   int recon;
   int flow_axis;           // streamwise axis of the flow-rate controller
-  // FP16 deviation codec: f_i = base_i + w_i * s_i, s_i = (f_i - base_i) / w_i
-  float hbase[Q];
-  float hw[Q];
-  float hinvw[Q];
+  // FP16 deviation codec: f_i = w_i (rho0 + s_i), s_i = f_i / w_i - rho0
+  float rho0f;
 };
 
 struct Ctx;

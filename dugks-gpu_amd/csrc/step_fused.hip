@@ -17,6 +17,8 @@
 //     src/dugks.cu:675-709, 1201-1207) and keeps only the populations that
 //     cross the face;
 //   - updates plane p (src/dugks.cu:945-956) into the other f buffer.
+// Two barriers per plane: [f-bar(p+1); write Z(p), Zn(p+1/2)] | [faces] |
+// [update], the update of one plane overlapping the next plane's f-bar.
 // z must be periodic; x and y may be periodic or walls, uniform or stretched.
 // Other configurations use the two-pass kernels.
 #include "device_common.h"
@@ -209,7 +211,8 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
                                                           double* __restrict__ partial) {
   using T = Tile<TX, TY>;
   constexpr int AC = T::AC;
-  __shared__ Real A[Q][TY + 2][AC];        // Z(p), then Zn(p+1/2)
+  __shared__ Real A[Q][TY + 2][AC];        // Z(p): tangential z-lerp of f-bar(p)
+  __shared__ Real An[Q][TY + 2][AC];       // Zn(p+1/2): normal z-lerp between p and p+1
   __shared__ Real Cx[10][TY][TX + 1];      // crossing populations of -x faces
   __shared__ Real Dy[10][TY + 1][TX];      // crossing populations of -y faces
 
@@ -260,15 +263,15 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
   double flow = 0;
   const int as = P.flow_axis;
 
-  // Zn(p+1/2) of this column into A: z-normal lerp between fb(p) and fb(p+1)
+  // Zn(p+1/2) of this column into An: z-normal lerp between fb(p) and fb(p+1)
   auto store_zn = [&](const Real (&lo)[Q], const Real (&hi)[Q]) {
     if (!active) return;
     static_for<Q>([&](auto I) {
       constexpr int ip = decltype(I)::value;
       if constexpr (ip == 0)
-        A[0][r][c + 1] = Real(0.5) * (lo[0] + hi[0]);
+        An[0][r][c + 1] = Real(0.5) * (lo[0] + hi[0]);
       else
-        A[ip][r][c + 1] = lerp_(lo[ip], hi[ip], wzn[CZ[ip] + 1]);
+        An[ip][r][c + 1] = lerp_(lo[ip], hi[ip], wzn[CZ[ip] + 1]);
     });
   };
   // +z face of an interior column from Zn in A: bilinear over the upwind
@@ -278,18 +281,18 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
     static_for<Q>([&](auto I) {
       constexpr int ip = decltype(I)::value;
       constexpr int cx = CX[ip], cy = CY[ip];
-      Real v = A[ip][r][c + 1];
+      Real v = An[ip][r][c + 1];
       if constexpr (cx != 0) {
         const Real w = tx(mi * 9 + (cx < 0 ? 6 : 7));
-        v = lerp_(v, A[ip][r][c + 1 - cx], w);
+        v = lerp_(v, An[ip][r][c + 1 - cx], w);
         if constexpr (cy != 0) {
           const Real wy = ty(mj * 9 + (cy < 0 ? 6 : 7));
-          const Real v2 = lerp_(A[ip][r - cy][c + 1], A[ip][r - cy][c + 1 - cx], w);
+          const Real v2 = lerp_(An[ip][r - cy][c + 1], An[ip][r - cy][c + 1 - cx], w);
           v = lerp_(v, v2, wy);
         }
       } else if constexpr (cy != 0) {
         const Real wy = ty(mj * 9 + (cy < 0 ? 6 : 7));
-        v = lerp_(v, A[ip][r - cy][c + 1], wy);
+        v = lerp_(v, An[ip][r - cy][c + 1], wy);
       }
       fc.template put<ip>(v);
     });
@@ -316,7 +319,7 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
   for (int p = z0; p < zend; p++) {
     const int kn = (p + 1) % nz;
     if (active) fbar_column<Real, Store>(P, bin, Nb, gi, gj, kn, Famb, fbn);
-    // ---- phase 1: Z(p) = tangential z-lerp of f-bar(p) ----
+    // ---- phase 1: Z(p) (tangential z-lerp of f-bar(p)) and Zn(p+1/2) ----
     if (active) {
       static_for<Q>([&](auto I) {
         constexpr int ip = decltype(I)::value;
@@ -331,8 +334,9 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
         }
       });
     }
+    store_zn(fbc, fbn);
     __syncthreads();
-    // ---- phase 2a: -x and -y faces of plane p ----
+    // ---- phase 2: the faces of plane p (-x, -y from Z; +z from Zn) ----
     const bool do_x = interior || (active && t >= T::NROW && c == TX && r >= 1 && r <= TY);
     const bool do_y = (t < T::NROW && r >= 1);  // interior rows and the top halo row
     if (do_x) {
@@ -391,25 +395,28 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
         Dy[k][r - 1][c] = out[k];
       });
     }
-    __syncthreads();
-    // ---- phase 2b: Zn(p+1/2) ----
-    store_zn(fbc, fbn);
-    __syncthreads();
-    // ---- phase 2c: +z face and update of plane p ----
+    Real zp[10];
+    Real fo[Q];
     if (interior) {
-      Real zp[10];
       zface(zp);
+      // f~(p) for the update, loaded before the barrier so the wait hides it
+      const unsigned vb = (((unsigned)p * ny + gj) * nx + gi) * (unsigned)sizeof(Store);
+      static_for<Q>([&](auto I) {
+        constexpr int ip = decltype(I)::value;
+        fo[ip] = ld_state<Real, Store>(bin, vb, ip * Nb, P, ip);
+      });
+    }
+    __syncthreads();
+    // ---- phase 3: update of plane p ----
+    if (interior) {
       const unsigned cell = ((unsigned)p * ny + gj) * nx + gi;
       const unsigned vb = cell * (unsigned)sizeof(Store);
       const Real updx = tx(mi * 9 + 8), updy = ty(mj * 9 + 8);
-      __builtin_amdgcn_sched_barrier(0);
-      // f~(p+1) = 4/3 f-bar+ - 1/3 f~ - sum_a (dt/dx_a) c_a (F_a+ - F_a-), one
-      // population at a time (fenced so the f reloads are not all hoisted)
+      // f~(p+1) = 4/3 f-bar+ - 1/3 f~ - sum_a (dt/dx_a) c_a (F_a+ - F_a-)
       Real fnew[Q];
       static_for<Q>([&](auto I) {
         constexpr int ip = decltype(I)::value;
-        const Real fo = ld_state<Real, Store>(bin, vb, ip * Nb, P, ip);
-        Real v = fbc[ip] * Real(4.0 / 3.0) - fo * Real(1.0 / 3.0);
+        Real v = fbc[ip] * Real(4.0 / 3.0) - fo[ip] * Real(1.0 / 3.0);
         constexpr int kx = cross_index<0>(ip), ky = cross_index<1>(ip), kz = cross_index<2>(ip);
         if constexpr (kx >= 0) v -= Real(CX[ip]) * updx * (Cx[kx][r - 1][c + 1] - Cx[kx][r - 1][c]);
         if constexpr (ky >= 0) v -= Real(CY[ip]) * updy * (Dy[ky][r][c] - Dy[ky][r - 1][c]);
@@ -419,7 +426,6 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
         }
         st_state<Real, Store>(bout, vb, ip * Nb, v, P, ip);
         fnew[ip] = v;
-        if constexpr (ip % 4 == 3) __builtin_amdgcn_sched_barrier(0);
       });
       if (partial) {
         // bulk velocity of the new state for the flow-rate controller
@@ -444,7 +450,8 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
         fbc[ip] = fbn[ip];
       });
     }
-    __syncthreads();
+    // no barrier: the next phase 1 writes A/An, which phase 3 does not read,
+    // and the next phase 2 writes Cx/Dy only after the next phase-1 barrier
   }
 
   if (partial) {

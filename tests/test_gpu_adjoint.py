@@ -1,0 +1,234 @@
+"""GPU adjoint (through the C ABI) against the FP64 CPU oracle adjoint and
+central finite differences.
+
+The oracle's adjoint is itself pinned by central differences in
+test_oracle.py.  Here: one adjoint step vs the oracle on the same state and
+cotangent; the loss functionals vs their definitions; the sqrt(T)-checkpointed
+forward+adjoint vs central differences of J in random control directions and
+vs the chained oracle adjoint.
+"""
+import numpy as np
+import pytest
+
+from conftest import C, random_state
+import dugks_amd
+from dugks_amd import KERNEL_FUSED, KERNEL_TWOPASS, PREC_FP32, PREC_FP64, configs
+from pyoracle import Oracle
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def _engine(text, precision=PREC_FP64, kernel=-1):
+    return dugks_amd.Engine(text, precision=precision, kernel=kernel,
+                            stream=torch.cuda.current_stream().cuda_stream)
+
+
+def _oracle(eng, text):
+    kv = dict(l.split("=", 1) for l in text.strip().splitlines())
+    per = tuple(int(kv.get(k, "1")) for k in ("periods_x", "periods_y", "periods_z"))
+    width = tuple(float(kv[k]) for k in ("xwidth", "ywidth", "zwidth"))
+    famb = np.array([float(kv.get(k, "0")) for k in ("ambient_fx", "ambient_fy", "ambient_fz")])
+    sp = [None if kv.get(f"mesh_{a}", "uniform") == "uniform" else eng.spacing(i)
+          for i, a in enumerate("xyz")]
+    p = eng.params
+    return Oracle(eng.shape, per, width, dt=p["dt"], visc=p["visc"], famb=famb, spacing=sp), famb
+
+
+def _soa(aos):  # (nz,ny,nx,q) -> (q,nz,ny,nx)
+    return np.ascontiguousarray(np.moveaxis(aos, -1, 0))
+
+
+def _aos(soa):
+    return np.ascontiguousarray(np.moveaxis(soa, 0, -1))
+
+
+CASES = {
+    "periodic": dict(nx=32, ny=8, nz=4, xwidth=3.2, ywidth=0.8, zwidth=0.4, ambient_fx=1e-4),
+    "walls_y": dict(nx=32, ny=16, nz=4, periods_y=0, ambient_fx=2e-4, xwidth=3.2, ywidth=1.6, zwidth=0.4),
+    "walls_x_stretched": dict(nx=16, ny=8, nz=4, periods_x=0, mesh_x="tanh:0.8", dt_rule="min",
+                              ambient_fz=2e-4, xwidth=2.0, ywidth=0.8, zwidth=0.4),
+    "walls_z": dict(nx=8, ny=6, nz=10, periods_z=0, ambient_fy=2e-4, xwidth=0.8, ywidth=0.6, zwidth=1.0),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("precision,tol", [(PREC_FP64, 1e-12), (PREC_FP32, 2e-5)], ids=["fp64", "fp32"])
+def test_adjoint_step_matches_oracle(rng, case, precision, tol):
+    text = configs.base(visc=0.02, **CASES[case])
+    eng = _engine(text, precision=precision)
+    ora, famb = _oracle(eng, text)
+    f0 = random_state(eng.shape, rng)
+    eng.set_f(f0)
+    f0 = eng.get_f()  # the stored (possibly rounded) state
+    nx, ny, nz = eng.shape
+    G = 1e-4 * rng.standard_normal((3, nz, ny, nx))
+    eng.set_cell_force(*G)
+    ora.set_f(f0)
+    ora.set_cell_force(*(G + famb.reshape(3, 1, 1, 1)))
+    lam = rng.standard_normal(f0.shape)
+    lp_ref, g_ref = ora.adjoint_step(lam)
+    dev = torch.device("cuda")
+    lam_d = torch.from_numpy(_soa(lam)).to(dev)
+    lp_d = torch.empty_like(lam_d)
+    g_d = torch.zeros((3, nz, ny, nx), dtype=torch.float64, device=dev)
+    eng.adjoint_step(lam_d, lp_d, g_d)
+    torch.cuda.synchronize()
+    lp = _aos(lp_d.cpu().numpy())
+    g = np.moveaxis(g_d.cpu().numpy(), 0, -1)
+    assert np.abs(lp - lp_ref).max() <= tol * np.abs(lp_ref).max()
+    assert np.abs(g - g_ref).max() <= tol * np.abs(g_ref).max()
+
+
+def _wall_shear_grad(eng, famb, nu):
+    """dJ/df of the mean wall shear stress (definition in include/dugks.h)."""
+    nx, ny, nz = eng.shape
+    h = eng.params["hdt"]
+    w = eng.spacing(1)
+    g = np.zeros((nz, ny, nx, 19))
+    for jw in (0, ny - 1):
+        coef = nu / (0.5 * w[jw] * 2.0 * nx * nz)
+        g[:, jw, :, :] = coef * (C[:, 0] + h * famb[0])
+    return g
+
+
+def test_loss_functionals_match_definitions(rng):
+    text = configs.adjoint_channel(nx=32, ny=16, nz=4)
+    eng = _engine(text)
+    f = eng.get_f()
+    famb = eng.ambient_force()
+    nu = eng.params["visc"]
+    g = _wall_shear_grad(eng, famb, nu)
+    dev = torch.device("cuda")
+    gd = torch.empty((19,) + f.shape[:3], dtype=torch.float64, device=dev)
+    J = eng.loss(dugks_amd.LOSS_WALL_SHEAR, gd)
+    torch.cuda.synchronize()
+    assert abs(J - (g * f).sum()) <= 1e-13 * abs(J)
+    assert np.abs(_aos(gd.cpu().numpy()) - g).max() <= 1e-15 * np.abs(g).max()
+    # kinetic energy 0.5 <rho |u|^2>, u = j/rho + h F
+    K = eng.loss(dugks_amd.LOSS_KINETIC_ENERGY, gd)
+    torch.cuda.synchronize()
+    rho = f.sum(-1)
+    u = np.einsum("kjiq,qa->kjia", f, C) / rho[..., None] + eng.params["hdt"] * famb
+    w = eng.spacing(1)
+    dV = (4 * np.pi / 32) * w[None, :, None] * (4 * np.pi / 3 / 4)
+    Kref = (0.5 * rho * (u ** 2).sum(-1) * dV).sum() / (4 * np.pi * 2 * 4 * np.pi / 3)
+    assert abs(K - Kref) <= 1e-12 * Kref
+
+
+def _run_J(eng, f0, G, T, kind=dugks_amd.LOSS_WALL_SHEAR):
+    eng.set_f(f0)
+    eng.set_cell_force(*G)
+    total = 0.0
+    for n in range(T):
+        eng.step(1, istep0=1 + n)
+        total += eng.loss(kind)
+    return total / T
+
+
+def _adjoint_problem(rng, precision, nx=32, ny=16, nz=8, T=24):
+    # transitional channel (seeded perturbed laminar state, Re = 2000),
+    # wall-shear loss averaged over T steps, control G ~ N(0, 1e-6)
+    text = configs.adjoint_channel(nx=nx, ny=ny, nz=nz)
+    eng = _engine(text, precision=precision)
+    f0 = eng.get_f()
+    G = 1e-6 * np.random.default_rng(99).standard_normal((3, nz, ny, nx))
+    return eng, f0, G, T
+
+
+def _grad(eng, f0, G, T, interval=0):
+    nx, ny, nz = eng.shape
+    eng.set_f(f0)
+    eng.set_cell_force(*G)
+    gd = torch.empty((3, nz, ny, nx), dtype=torch.float64, device=torch.device("cuda"))
+    J, peak = eng.forward_adjoint(T, gd, interval=interval)
+    torch.cuda.synchronize()
+    return J, gd.cpu().numpy(), peak
+
+
+def test_forward_adjoint_matches_central_differences(rng):
+    eng, f0, G, T = _adjoint_problem(rng, PREC_FP64)
+    J, grad, _ = _grad(eng, f0, G, T)
+    # recomputation from checkpoints is exact: every interval gives the same gradient
+    for interval in (1, 5, T):
+        Jk, gk, _ = _grad(eng, f0, G, T, interval)
+        assert Jk == J and (gk == grad).all()
+    assert abs(J - _run_J(eng, f0, G, T)) <= 1e-12 * abs(J)
+    # perturbations of the control's own size (1e-6): the response is linear
+    # to O(1e-12) there, and J's round-off stays far below the difference
+    eps = 1.0
+    for trial in range(4):
+        v = rng.standard_normal(G.shape) * 1e-6
+        jp = _run_J(eng, f0, G + eps * v, T)
+        jm = _run_J(eng, f0, G - eps * v, T)
+        fd = (jp - jm) / (2 * eps)
+        an = float((grad * v).sum())
+        assert abs(fd - an) <= 1e-6 * abs(an), (trial, fd, an)
+
+
+def test_fp32_gradient_against_fp64(rng):
+    e64, f64, G, T = _adjoint_problem(rng, PREC_FP64)
+    J64, g64, _ = _grad(e64, f64, G, T)
+    e32, f32, _, _ = _adjoint_problem(rng, PREC_FP32)
+    J32, g32, _ = _grad(e32, f32, G, T)
+    assert abs(J32 - J64) <= 1e-5 * abs(J64)
+    assert np.abs(g32 - g64).max() <= 1e-3 * np.abs(g64).max()
+
+
+def test_forward_adjoint_matches_chained_oracle_adjoint(rng):
+    text = configs.adjoint_channel(nx=16, ny=16, nz=4)
+    eng = _engine(text)
+    ora, famb = _oracle(eng, text)
+    f0 = eng.get_f()
+    nx, ny, nz = eng.shape
+    G = 1e-6 * rng.standard_normal((3, nz, ny, nx))
+    T = 10
+    dev = torch.device("cuda")
+    gd = torch.empty((3, nz, ny, nx), dtype=torch.float64, device=dev)
+    g0 = torch.empty((19, nz, ny, nx), dtype=torch.float64, device=dev)
+    eng.set_cell_force(*G)
+    J, _ = eng.forward_adjoint(T, gd, interval=3, grad_f0=g0)
+    torch.cuda.synchronize()
+    # oracle: forward with stored states, then the chained one-step adjoints
+    ora.set_cell_force(*(G + famb.reshape(3, 1, 1, 1)))
+    states = [f0]
+    ora.set_f(f0)
+    for n in range(T):
+        ora.step(1)
+        states.append(ora.get_f())
+    dJ = _wall_shear_grad(eng, famb, eng.params["visc"]) / T
+    Jref = sum((dJ * s).sum() for s in states[1:])
+    assert abs(J - Jref) <= 1e-12 * abs(Jref)
+    lam = np.zeros_like(f0)
+    gF = np.zeros((nz, ny, nx, 3))
+    for n in range(T - 1, -1, -1):
+        lam = lam + dJ
+        ora.set_f(states[n])
+        lam, gF = ora.adjoint_step(lam, gF)
+    grad = np.moveaxis(gd.cpu().numpy(), 0, -1)
+    assert np.abs(grad - gF).max() <= 1e-11 * np.abs(gF).max()
+    assert np.abs(_aos(g0.cpu().numpy()) - lam).max() <= 1e-11 * np.abs(lam).max()
+
+
+def test_autograd_function(rng):
+    from dugks_amd.autograd import control_loss
+    text = configs.adjoint_channel(nx=32, ny=16, nz=4)
+    eng = _engine(text)
+    nx, ny, nz = eng.shape
+    dev = torch.device("cuda")
+    f0 = torch.empty((19, nz, ny, nx), dtype=torch.float64, device=dev)
+    eng.get_f_device(f0)
+    G = (1e-6 * torch.randn((3, nz, ny, nx), dtype=torch.float64, device=dev)).requires_grad_(True)
+    J = control_loss(G, eng, f0, 8)
+    J.backward()
+    g1 = G.grad.clone()
+    gd = torch.empty_like(g1)
+    eng.set_f_device(f0)
+    eng.set_cell_force_device(G.detach())
+    J2, _ = eng.forward_adjoint(8, gd)
+    torch.cuda.synchronize()
+    assert abs(J.item() - J2) <= 1e-14 * abs(J2)
+    assert torch.equal(g1, gd)
+    with torch.no_grad():
+        J3 = control_loss(G.detach(), eng, f0, 8)
+    assert abs(J3.item() - J2) <= 1e-13 * abs(J2)
