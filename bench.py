@@ -28,7 +28,37 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_mode(dugks_amd, configs, torch, precision, args):
+def field_stats(eng, np):
+    """rho, u and the wall-normal (y) profiles of mean u_x, rms velocities and
+    <u'v'> of the current state, FP64 on the host."""
+    rho, ux, uy, uz = eng.macro()  # [k][j][i]
+    prof = {}
+    mx = ux.mean(axis=(0, 2))
+    prof["U"] = mx
+    for name, a in (("urms", ux), ("vrms", uy), ("wrms", uz)):
+        prof[name] = np.sqrt(np.maximum((a * a).mean(axis=(0, 2)) - a.mean(axis=(0, 2)) ** 2, 0))
+    prof["uv"] = (ux * uy).mean(axis=(0, 2)) - mx * uy.mean(axis=(0, 2))
+    return dict(rho=rho, u=(ux, uy, uz), prof=prof)
+
+
+def field_errors(ref, got, np):
+    """Errors of a reduced-precision state against the FP64 state at the same
+    step: max and RMS of rho and |u| (relative to the FP64 maxima) and of the
+    y-profiles (relative to each FP64 profile's maximum)."""
+    d_rho = got["rho"] - ref["rho"]
+    du2 = sum((g - r) ** 2 for g, r in zip(got["u"], ref["u"]))
+    umax = float(np.sqrt(sum(r * r for r in ref["u"]).max()))
+    out = {"rho_max": float(np.abs(d_rho).max() / np.abs(ref["rho"]).max()),
+           "rho_rms": float(np.sqrt((d_rho ** 2).mean()) / np.abs(ref["rho"]).max()),
+           "u_max": float(np.sqrt(du2.max()) / umax),
+           "u_rms": float(np.sqrt(du2.mean()) / umax)}
+    for k in ("U", "urms", "vrms", "wrms", "uv"):
+        scale = float(np.abs(ref["prof"][k]).max())
+        out[f"{k}_profile_max"] = float(np.abs(got["prof"][k] - ref["prof"][k]).max() / scale) if scale else 0.0
+    return out
+
+
+def run_mode(dugks_amd, configs, torch, precision, args, np=None):
     prec = {"fp64": dugks_amd.PREC_FP64, "fp32": dugks_amd.PREC_FP32,
             "fp16": dugks_amd.PREC_FP16S}[precision]
     kernel = {"auto": dugks_amd.KERNEL_AUTO, "fused": dugks_amd.KERNEL_FUSED,
@@ -56,12 +86,69 @@ def run_mode(dugks_amd, configs, torch, precision, args):
     achieved = alg_bytes / (kt["mean_ms"] * 1e-3) / 1e9
     s = eng.global_sums()
     ok = all(math.isfinite(v) for v in s) and s[0] > 0
+    stats = field_stats(eng, np) if np is not None else None
     res = dict(mlups=mlups, ms_per_step=wall * 1e3 / args.steps, kernel_ms=kt["mean_ms"],
                kernel_ms_min=kt["min_ms"], launches_per_step=int(kt["launches_per_step"]),
                alg_bytes_per_launch=alg_bytes, achieved_gbs=achieved,
                hbm_frac=achieved / HBM_PEAK_GBS, launches=int(kt["launches_per_step"]), finite=bool(ok))
     log(f"{precision}: {mlups:.0f} MLUPS, kernel {kt['mean_ms']:.3f} ms, {achieved:.0f} GB/s algorithmic")
     eng.close()
+    return res, stats
+
+
+def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32"):
+    """Forward + adjoint over T steps of the same 256x192x256 channel with a
+    constant driving force (the adjoint does not differentiate the flow-rate
+    controller): wall-shear loss, sqrt(T) checkpoints in HBM, gradient with
+    respect to a per-cell body force.  Reports the whole-sweep and per-step
+    adjoint/forward time ratios and the sweep's device memory."""
+    prec = {"fp64": dugks_amd.PREC_FP64, "fp32": dugks_amd.PREC_FP32}[precision]
+    text = configs.large_channel(precision=precision, flow_rate=0)
+    eng = dugks_amd.Engine(text, precision=prec, device=0, stream=torch.cuda.current_stream().cuda_stream)
+    nz, ny, nx = eng.shape[2], eng.shape[1], eng.shape[0]
+    f0 = torch.empty((19, nz, ny, nx), dtype=torch.float64, device="cuda")
+    eng.get_f_device(f0)
+    eng.step(2, istep0=1)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    eng.step(T, istep0=3)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t
+    # one adjoint step (recompute f-bar+ and ghosts, face cotangents, gather)
+    lam = torch.randn((19, nz, ny, nx), dtype=torch.float64, device="cuda")
+    lam2 = torch.empty_like(lam)
+    g = torch.zeros((3, nz, ny, nx), dtype=torch.float64, device="cuda")
+    eng.adjoint_step(lam, lam2, g)
+    torch.cuda.synchronize()
+    nadj = 10
+    t = time.perf_counter()
+    for _ in range(nadj):
+        eng.adjoint_step(lam, lam2, g)
+    torch.cuda.synchronize()
+    t_adj = (time.perf_counter() - t) / nadj
+    del lam, lam2
+    eng.set_f_device(f0)
+    del f0
+    torch.cuda.synchronize()
+    free0, total = torch.cuda.mem_get_info()
+    t = time.perf_counter()
+    J, peak = eng.forward_adjoint(T, g)
+    torch.cuda.synchronize()
+    t_fa = time.perf_counter() - t
+    gmax = float(g.abs().max())
+    ok = math.isfinite(J) and math.isfinite(gmax)
+    eng.close()
+    fwd_step = t_fwd / T
+    res = dict(precision=precision, T=T, interval=int(math.ceil(math.sqrt(T))), loss="wall shear",
+               forward_ms_per_step=round(fwd_step * 1e3, 4),
+               adjoint_step_ms=round(t_adj * 1e3, 4),
+               adjoint_over_forward_step=round(t_adj / fwd_step, 3),
+               fwd_adj_sweep_s=round(t_fa, 3),
+               sweep_over_forward=round(t_fa / t_fwd, 3),
+               sweep_device_bytes=peak, device_free_before=free0, device_total=total,
+               finite=bool(ok))
+    log(f"adjoint: step {t_adj*1e3:.2f} ms vs forward {fwd_step*1e3:.3f} ms; "
+        f"{T}-step sweep {t_fa:.2f} s ({t_fa/t_fwd:.1f}x forward), {peak/1e9:.1f} GB")
     return res
 
 
@@ -100,6 +187,8 @@ def main():
     ap.add_argument("--kernel", default="auto", choices=["auto", "fused", "twopass"])
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-errors", action="store_true")
+    ap.add_argument("--adj-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the domain is not sharded); "
@@ -109,10 +198,25 @@ def main():
     import dugks_amd
     from dugks_amd import configs
 
+    import numpy as np
     modes = [m for m in args.modes.split(",") if m]
-    results = {}
+    results, stats = {}, {}
+    want_err = "fp64" in modes and len(modes) > 1 and not args.no_errors
     for m in modes:
-        results[m] = run_mode(dugks_amd, configs, torch, m, args)
+        results[m], stats[m] = run_mode(dugks_amd, configs, torch, m, args, np if want_err else None)
+    errors = None
+    if want_err:
+        nstep = args.warmup + args.steps + min(args.steps, 50)
+        errors = {"steps": nstep, "against": "fp64"}
+        for m in modes:
+            if m != "fp64":
+                errors[m] = {k: float(f"{v:.3e}") for k, v in field_errors(stats["fp64"], stats[m], np).items()}
+                log(f"{m} vs fp64 after {nstep} steps: u max {errors[m]['u_max']:.2e}, "
+                    f"U profile {errors[m]['U_profile_max']:.2e}")
+    stats.clear()
+    adjoint = None
+    if args.adj_steps > 0:
+        adjoint = adjoint_leg(dugks_amd, configs, torch, args.adj_steps)
     primary = "fp32" if "fp32" in results else modes[0]
     r = results[primary]
     cpu = None
@@ -142,6 +246,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": cpu,
+        "precision_errors": errors,
+        "adjoint": adjoint,
         "modes": {m: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res.items()}
                   for m, res in results.items()},
     }

@@ -215,6 +215,8 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
   __shared__ Real An[Q][TY + 2][AC];       // Zn(p+1/2): normal z-lerp between p and p+1
   __shared__ Real Cx[10][TY][TX + 1];      // crossing populations of -x faces
   __shared__ Real Dy[10][TY + 1][TX];      // crossing populations of -y faces
+  __shared__ Real WX[6][TX + 2];           // weights per tile column
+  __shared__ Real WY[6][TY + 2];           // weights per tile row
 
   const int nx = P.n[0], ny = P.n[1], nz = P.n[2];
   const int t = threadIdx.x;
@@ -258,6 +260,29 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
   const Real wzt_p = tz(9 + 7);  // tangential, toward p-1 (c_z = +1)
   const Real wzn[3] = {tz(9 + 0), tz(9 + 1), tz(9 + 2)};  // normal, toward p+1
   const Real updz = tz(9 + 8);
+  // Per-column and per-row weights of the tile, copied once into LDS: the
+  // tables are not written by this kernel, but the compiler cannot prove that
+  // against the f stores and would reload them (or hold them in registers)
+  // across the plane loop.  Row k of WX/WY: 0-2 the -face normal weight
+  // toward the upper cell for c = -1, 0, +1; 3-4 the tangential shift
+  // weights for c < 0 and c > 0; 5 dt/d.  The -face index is clamped for the
+  // ghost column/row, which never owns a -face.
+  for (int u = t; u < 6 * (TX + 2) + 6 * (TY + 2); u += T::NT) {
+    const bool isx = u < 6 * (TX + 2);
+    const int v = isx ? u : u - 6 * (TX + 2);
+    const int n = isx ? TX + 2 : TY + 2;
+    const int k = v / n, q = v % n;
+    int g = isx ? i0 - 1 + q : j0 - 1 + q;
+    const int na = isx ? nx : ny;
+    if (P.periodic[isx ? 0 : 1]) g = (g + na) % na;
+    const int m = g + 1;
+    const int idx = k < 3 ? max(m - 1, 0) * 9 + k : m * 9 + (k + 3);
+    (isx ? WX[k][q] : WY[k][q]) = isx ? tx(idx) : ty(idx);
+  }
+  auto xn = [&](int k) { return WX[k][c + 1]; };
+  auto xt = [&](int s) { return WX[3 + s][c + 1]; };
+  auto yn = [&](int k) { return WY[k][r]; };
+  auto yt = [&](int s) { return WY[3 + s][r]; };
 
   Real fbm[5], fbc[Q], fbn[Q], zm[10];
   double flow = 0;
@@ -283,15 +308,15 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
       constexpr int cx = CX[ip], cy = CY[ip];
       Real v = An[ip][r][c + 1];
       if constexpr (cx != 0) {
-        const Real w = tx(mi * 9 + (cx < 0 ? 6 : 7));
+        const Real w = xt(cx < 0 ? 0 : 1);
         v = lerp_(v, An[ip][r][c + 1 - cx], w);
         if constexpr (cy != 0) {
-          const Real wy = ty(mj * 9 + (cy < 0 ? 6 : 7));
+          const Real wy = yt(cy < 0 ? 0 : 1);
           const Real v2 = lerp_(An[ip][r - cy][c + 1], An[ip][r - cy][c + 1 - cx], w);
           v = lerp_(v, v2, wy);
         }
       } else if constexpr (cy != 0) {
-        const Real wy = ty(mj * 9 + (cy < 0 ? 6 : 7));
+        const Real wy = yt(cy < 0 ? 0 : 1);
         v = lerp_(v, An[ip][r - cy][c + 1], wy);
       }
       fc.template put<ip>(v);
@@ -342,17 +367,16 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
     if (do_x) {
       // face between x-columns gi-1 (left, ghost-inclusive index mi-1) and gi
       Face<Real, 0> fc;
-      const int tl = (mi - 1) * 9;
       static_for<Q>([&](auto I) {
         constexpr int ip = decltype(I)::value;
         constexpr int cx = CX[ip], cy = CY[ip];
         if constexpr (ip == 0) {
           fc.template put<0>(Real(0.5) * (A[0][r][c] + A[0][r][c + 1]));
         } else {
-          const Real w = tx(tl + cx + 1);  // fraction toward the right cell
+          const Real w = xn(cx + 1);  // fraction toward the right cell
           Real v = lerp_(A[ip][r][c], A[ip][r][c + 1], w);
           if constexpr (cy != 0) {
-            const Real wy = ty(mj * 9 + (cy < 0 ? 6 : 7));
+            const Real wy = yt(cy < 0 ? 0 : 1);
             const Real v2 = lerp_(A[ip][r - cy][c], A[ip][r - cy][c + 1], w);
             v = lerp_(v, v2, wy);
           }
@@ -370,17 +394,16 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
     if (do_y) {
       // face between y-rows gj-1 (below) and gj
       Face<Real, 1> fc;
-      const int tb = (mj - 1) * 9;
       static_for<Q>([&](auto I) {
         constexpr int ip = decltype(I)::value;
         constexpr int cx = CX[ip], cy = CY[ip];
         if constexpr (ip == 0) {
           fc.template put<0>(Real(0.5) * (A[0][r - 1][c + 1] + A[0][r][c + 1]));
         } else {
-          const Real w = ty(tb + cy + 1);  // fraction toward the upper cell
+          const Real w = yn(cy + 1);  // fraction toward the upper cell
           Real lo = A[ip][r - 1][c + 1], hi = A[ip][r][c + 1];
           if constexpr (cx != 0) {
-            const Real wx = tx(mi * 9 + (cx < 0 ? 6 : 7));
+            const Real wx = xt(cx < 0 ? 0 : 1);
             lo = lerp_(lo, A[ip][r - 1][c + 1 - cx], wx);
             hi = lerp_(hi, A[ip][r][c + 1 - cx], wx);
           }
@@ -396,27 +419,19 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
       });
     }
     Real zp[10];
-    Real fo[Q];
-    if (interior) {
-      zface(zp);
-      // f~(p) for the update, loaded before the barrier so the wait hides it
-      const unsigned vb = (((unsigned)p * ny + gj) * nx + gi) * (unsigned)sizeof(Store);
-      static_for<Q>([&](auto I) {
-        constexpr int ip = decltype(I)::value;
-        fo[ip] = ld_state<Real, Store>(bin, vb, ip * Nb, P, ip);
-      });
-    }
+    if (interior) zface(zp);
     __syncthreads();
     // ---- phase 3: update of plane p ----
     if (interior) {
       const unsigned cell = ((unsigned)p * ny + gj) * nx + gi;
       const unsigned vb = cell * (unsigned)sizeof(Store);
-      const Real updx = tx(mi * 9 + 8), updy = ty(mj * 9 + 8);
+      const Real updx = WX[5][c + 1], updy = WY[5][r];
       // f~(p+1) = 4/3 f-bar+ - 1/3 f~ - sum_a (dt/dx_a) c_a (F_a+ - F_a-)
       Real fnew[Q];
       static_for<Q>([&](auto I) {
         constexpr int ip = decltype(I)::value;
-        Real v = fbc[ip] * Real(4.0 / 3.0) - fo[ip] * Real(1.0 / 3.0);
+        const Real fo = ld_state<Real, Store>(bin, vb, ip * Nb, P, ip);
+        Real v = fbc[ip] * Real(4.0 / 3.0) - fo * Real(1.0 / 3.0);
         constexpr int kx = cross_index<0>(ip), ky = cross_index<1>(ip), kz = cross_index<2>(ip);
         if constexpr (kx >= 0) v -= Real(CX[ip]) * updx * (Cx[kx][r - 1][c + 1] - Cx[kx][r - 1][c]);
         if constexpr (ky >= 0) v -= Real(CY[ip]) * updy * (Dy[ky][r][c] - Dy[ky][r - 1][c]);
@@ -429,15 +444,17 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
       });
       if (partial) {
         // bulk velocity of the new state for the flow-rate controller
-        double rr = 0, js = 0;
+        // (per-cell moments in the arithmetic type, the sum over cells in FP64)
+        Real rr = 0, jj[3] = {0, 0, 0};
         static_for<Q>([&](auto I) {
           constexpr int ip = decltype(I)::value;
-          rr += (double)fnew[ip];
-          js += (double)cv(0, ip) * (as == 0) * (double)fnew[ip] + (double)cv(1, ip) * (as == 1) * (double)fnew[ip] +
-                (double)cv(2, ip) * (as == 2) * (double)fnew[ip];
+          rr += fnew[ip];
+          if constexpr (CX[ip] != 0) jj[0] += CX[ip] > 0 ? fnew[ip] : -fnew[ip];
+          if constexpr (CY[ip] != 0) jj[1] += CY[ip] > 0 ? fnew[ip] : -fnew[ip];
+          if constexpr (CZ[ip] != 0) jj[2] += CZ[ip] > 0 ? fnew[ip] : -fnew[ip];
         });
-        const double dV = P.wd[0][mi] * P.wd[1][mj] * P.wd[2][p + 1];
-        flow += js / rr * dV;  // momentum bulk velocity, see k_flow_control
+        const Real js = as == 0 ? jj[0] : (as == 1 ? jj[1] : jj[2]);
+        flow += (double)(js * rcp_real(rr));  // u along the flow axis; see k_flow_control
       }
     }
     if (active) {
@@ -457,7 +474,8 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
   if (partial) {
     // fixed-order block reduction of the flow sums
     double* red = reinterpret_cast<double*>(&A[0][0][0]);
-    red[t] = flow;
+    // every column of the tile has the same z width (z is uniform here)
+    red[t] = interior ? flow * (P.wd[0][mi] * P.wd[1][mj] * P.wd[2][1]) : 0.0;
     __syncthreads();
     for (int s = 1; s < T::NT; s <<= 1) {
       const int s2 = s << 1;
@@ -527,7 +545,10 @@ bool fused_supported(const Ctx& c, int* tx, int* ty) {
   if (s.recon != 0 || !s.periodic[2] || !s.uniform[2]) return false;
   // buffer offsets are 32-bit: one state array must span < 4 GiB
   if ((double)s.ncell() * Q * c.store_bytes >= 4294967296.0) return false;
-  *tx = (c.precision != 0 && s.n[0] % 64 == 0) ? 64 : 32;
+  // 32-wide tiles: the 64-wide tile's 11 waves cap registers at 168 per
+  // lane and the kernel then spills, which costs more than the wider tile's
+  // smaller halo saves
+  *tx = 32;
   *ty = 8;
   if (s.n[0] % *tx != 0 || s.n[1] % *ty != 0) return false;
   // a wall axis needs two interior cells per side of the extrapolation
@@ -541,14 +562,13 @@ hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled)
   *handled = false;
   if (!fused_supported(c, &tx, &ty)) return hipSuccess;
   *handled = true;
-  const bool wide = tx == 64;
   switch (c.precision) {
     case 0:
       return run_fused<double, double, 32, 8>(c, fin, fout);
     case 1:
-      return wide ? run_fused<float, float, 64, 8>(c, fin, fout) : run_fused<float, float, 32, 8>(c, fin, fout);
+      return run_fused<float, float, 32, 8>(c, fin, fout);
     default:
-      return wide ? run_fused<float, __half, 64, 8>(c, fin, fout) : run_fused<float, __half, 32, 8>(c, fin, fout);
+      return run_fused<float, __half, 32, 8>(c, fin, fout);
   }
 }
 
