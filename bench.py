@@ -24,6 +24,21 @@ BYTES_PER_VALUE = {"fp64": 8, "fp32": 4, "fp16": 2}
 METRIC = "MLUPS & %HBM peak: FP64 vs FP32/FP16-store at stated error; adjoint/fwd cost"
 
 
+def measured_traffic(precision):
+    """HBM bytes per launch of the step kernel on this workload, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary committed under profiles/
+    (each counter scaled by its reading for a copy kernel of known size).
+    None when no summary for this precision exists."""
+    path = os.path.join(ROOT, "profiles", "k_fused_traffic_256x192x256.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 3 and parts[0] == "traffic" and parts[1] == precision:
+            return int(float(parts[2]))
+    return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -189,12 +204,24 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--adj-steps", type=int, default=100)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB read + 1 GiB write elementwise kernel 5 times "
+                         "(the reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the domain is not sharded); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
+    if args.copy_calibration:
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        y = torch.empty_like(x)
+        for _ in range(5):
+            torch.add(x, 0.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"read_bytes": x.numel() * 4, "write_bytes": y.numel() * 4,
+                                               "launches": 5}}), flush=True)
+        return
     import dugks_amd
     from dugks_amd import configs
 
@@ -244,7 +271,8 @@ def main():
                    "step_kernel": "fused" if r["launches_per_step"] <= 2 else "twopass"},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(primary)},
         "cpu_baseline": cpu,
         "precision_errors": errors,
         "adjoint": adjoint,

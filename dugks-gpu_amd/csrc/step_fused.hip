@@ -38,6 +38,8 @@ __device__ constexpr int dirs(int k) {
   return A == 0 ? XD[k] : (A == 1 ? YD[k] : ZD[k]);
 }
 
+constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
+
 template <int TX, int TY>
 struct Tile {
   static constexpr int NROW = (TY + 2) * TX;
@@ -220,8 +222,19 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
 
   const int nx = P.n[0], ny = P.n[1], nz = P.n[2];
   const int t = threadIdx.x;
-  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
-  const int z0 = blockIdx.z * TZ;
+  // XCD-aware tile order.  Workgroups are dealt round-robin to the 8 XCDs
+  // (workgroup b runs on XCD b % 8), each with its own L2.  Tile tt =
+  // (b % 8) * (ntiles / 8) + b / 8 gives every XCD a contiguous run of tiles
+  // (x fastest), so the tiles whose halos overlap share one L2 and run at
+  // about the same time; the plain order would put x-neighbours on
+  // different XCDs and every x-halo line would come from HBM.
+  const int gx = nx / TX, gy = ny / TY;
+  const int ntiles = gx * gy * (int)gridDim.z;
+  const int b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const int tt = (ntiles % kXcds == 0) ? (b % kXcds) * (ntiles / kXcds) + b / kXcds : b;
+  const int bx = tt % gx, by = (tt / gx) % gy, bz = tt / (gx * gy);
+  const int i0 = bx * TX, j0 = by * TY;
+  const int z0 = bz * TZ;
   const int zend = min(z0 + TZ, nz);
   const unsigned Nb = (unsigned)(P.N * (long long)sizeof(Store));  // bytes per population
   const StateBuf<Store> bin(fin, Q * Nb), bout(fout, Q * Nb);
@@ -483,7 +496,7 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
       __syncthreads();
     }
     if (t == 0)
-      partial[((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+      partial[tt] = red[0];
   }
 }
 
@@ -549,7 +562,9 @@ bool fused_supported(const Ctx& c, int* tx, int* ty) {
   // lane and the kernel then spills, which costs more than the wider tile's
   // smaller halo saves
   *tx = 32;
-  *ty = 8;
+  // FP64 state needs twice the registers: a 4-row tile (4 waves, one per
+  // SIMD) runs without spills, an 8-row tile (6 waves) spills
+  *ty = c.precision == 0 ? 4 : 8;
   if (s.n[0] % *tx != 0 || s.n[1] % *ty != 0) return false;
   // a wall axis needs two interior cells per side of the extrapolation
   if (!s.periodic[0] && s.n[0] < 3) return false;
@@ -558,13 +573,13 @@ bool fused_supported(const Ctx& c, int* tx, int* ty) {
 }
 
 hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled) {
-  int tx = 32, ty = 8;
+  int tx = 32, ty = 4;
   *handled = false;
   if (!fused_supported(c, &tx, &ty)) return hipSuccess;
   *handled = true;
   switch (c.precision) {
     case 0:
-      return run_fused<double, double, 32, 8>(c, fin, fout);
+      return run_fused<double, double, 32, 4>(c, fin, fout);
     case 1:
       return run_fused<float, float, 32, 8>(c, fin, fout);
     default:

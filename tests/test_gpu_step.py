@@ -58,6 +58,10 @@ CASES = {
                         ambient_fy=2e-4, xwidth=2.0, ywidth=0.8, zwidth=0.4),
     "force_all": dict(nx=32, ny=8, nz=4, ambient_fx=1e-4, ambient_fy=-2e-4, ambient_fz=3e-4,
                       xwidth=3.2, ywidth=0.8, zwidth=0.4),
+    # a multiple of 8 tiles (the XCD-ordered tile mapping) with a partial
+    # last z chunk (40 = 32 + 8)
+    "tiles_xcd": dict(nx=64, ny=16, nz=40, periods_y=0, mesh_y="tanh:0.85", dt_rule="min",
+                      ambient_fx=2e-4, xwidth=6.4, ywidth=2.0, zwidth=4.0),
 }
 # configurations only the two-pass kernels cover
 TWOPASS_CASES = {
