@@ -12,11 +12,12 @@
 //   2. k_adj_faces: every face once: S3 forward, S4^T driven by the face's
 //      cotangent lam_ft_q = c_qa (dt/dx_R lam'_q(R) - dt/dx_L lam'_q(L)) of
 //      its crossing populations.  The collision cotangent of every population
-//      is lam_fbar_q = fw1 lam_ft_q + A + B.c_q, so a face stores only the
-//      four scalars (A, B);
-//   3. k_adj_cells: per cell, S3^T as a gather over the faces whose stencil
-//      contains the cell (bounce-back and the wall ghost extrapolation folded
-//      in), then S5^T and S1^T.
+//      is lam_fbar_q = fw1 lam_ft_q + A + B.c_q; after the transpose of the
+//      wall bounce-back the face stores the cotangent of its 19
+//      reconstructed values (in the arithmetic type);
+//   3. k_adj_cells: per cell, S3^T as a gather of those values over the
+//      faces whose stencil contains the cell (the wall ghost extrapolation
+//      folded in), then S5^T and S1^T.
 // Deterministic (no atomics).  At most one wall axis.
 #include "device_common.h"
 
@@ -28,48 +29,6 @@ template <class Real>
 __device__ __forceinline__ Real lerp_(Real a, Real b, Real m) {
   return (Real(1) - m) * a + m * b;
 }
-
-// d/d(rho, u, F) of sum_i lam_i (B + C T_i) feq_i(rho, u), T_i = (c_i - u).F/RT,
-// over the populations in the compile-time set Sel (Sel(ip) true).
-template <class Real, class Sel>
-__device__ __forceinline__ void feq_adjoint(const Real (&lam)[Q], Real B, Real C, Real rho, const Real u[3],
-                                            const Real F[3], Real& g_rho, Real g_u[3], Real g_F[3], Sel sel) {
-  const Real uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-  const Real uF = u[0] * F[0] + u[1] * F[1] + u[2] * F[2];
-  g_rho = 0;
-  for (int a = 0; a < 3; a++) g_u[a] = g_F[a] = 0;
-  static_for<Q>([&](auto I) {
-    constexpr int ip = decltype(I)::value;
-    if constexpr (decltype(sel(I))::value) {
-      const Real c[3] = {Real(CX[ip]), Real(CY[ip]), Real(CZ[ip])};
-      const Real cu = c[0] * u[0] + c[1] * u[1] + c[2] * u[2];
-      const Real eu = Real(3) * cu;
-      const Real poly = Real(1) + eu + Real(0.5) * (eu * eu - Real(3) * uu);
-      const Real w = Real(wgt(ip));
-      const Real feq = w * rho * poly;
-      const Real cF = c[0] * F[0] + c[1] * F[1] + c[2] * F[2];
-      const Real T = Real(3) * (cF - uF);
-      const Real coef = B + C * T;
-      const Real l = lam[ip];
-      g_rho += l * coef * w * poly;
-      for (int a = 0; a < 3; a++) {
-        const Real dfeq = w * rho * Real(3) * (c[a] + eu * c[a] - u[a]);
-        g_u[a] += l * (coef * dfeq - C * Real(3) * F[a] * feq);
-        g_F[a] += l * C * Real(3) * (c[a] - u[a]) * feq;
-      }
-    }
-  });
-}
-
-struct AllPops {
-  template <class I>
-  __device__ constexpr std::true_type operator()(I) const { return {}; }
-};
-template <int A>
-struct Crossing {
-  template <class I>
-  __device__ constexpr std::integral_constant<bool, cv(A, I::value) != 0> operator()(I) const { return {}; }
-};
 
 __device__ __forceinline__ long long ext_index(const KParams& P, int m0, int m1, int m2) {
   return ((long long)m2 * P.e[1] + m1) * P.e[0] + m0;
@@ -91,28 +50,10 @@ __device__ __forceinline__ double lam_at(const KParams& P, const double* lam, in
   return lam[ip * P.N + c];
 }
 
-// Slot (left-cell index) of the face between m-1.. on axis a whose left cell
-// is mL, or -1 if that face does not exist.  Periodic axes store faces
-// 1..n (the face between 0 and 1 is the face between n and n+1); wall axes
-// store 0..n.
-__device__ __forceinline__ int face_slot(const KParams& P, int a, int mL) {
-  const int n = P.n[a];
-  if (P.periodic[a]) {
-    if (mL == 0) return n;
-    return (mL >= 1 && mL <= n) ? mL : -1;
-  }
-  return (mL >= 0 && mL <= n) ? mL : -1;
-}
-
-__device__ __forceinline__ int tangential_slot(const KParams& P, int b, int m) {
-  m = wrap(P, b, m);
-  return (m >= 1 && m <= P.n[b]) ? m : -1;
-}
-
 // ---- kernel 2: one thread per face of axis A ----
 template <class Real, int A>
 __global__ void __launch_bounds__(256) k_adj_faces(KParams P, const Real* __restrict__ fbx,
-                                                   const double* __restrict__ lam, double* __restrict__ faces) {
+                                                   const double* __restrict__ lam, Real* __restrict__ faces) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= P.eN) return;
   int m[3];
@@ -198,113 +139,125 @@ __global__ void __launch_bounds__(256) k_adj_faces(KParams P, const Real* __rest
   feq_adjoint<Real>(lft, (Real)P.fw2, (Real)P.fw0, rho, u, F, g_rho, g_u, g_F, Crossing<A>{});
   const Real invr = Real(1) / rho;
   const Real Acoef = g_rho - (g_u[0] * ut[0] + g_u[1] * ut[1] + g_u[2] * ut[2]) * invr;
-  double* out = faces + (long long)A * 4 * P.eN;
-  out[e] = (double)Acoef;
-  out[P.eN + e] = (double)(g_u[0] * invr);
-  out[2 * P.eN + e] = (double)(g_u[1] * invr);
-  out[3 * P.eN + e] = (double)(g_u[2] * invr);
+  const Real Bx = g_u[0] * invr, By = g_u[1] * invr, Bz = g_u[2] * invr;
+  // collision cotangent of each (post-bounce) face value
+  Real lpost[Q];
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    lpost[ip] = (Real)P.fw1 * lft[ip] + Acoef + Bx * Real(CX[ip]) + By * Real(CY[ip]) + Bz * Real(CZ[ip]);
+  });
+  // transpose of the bounce-back: an incoming population took its
+  // opposite's reconstructed value
+  Real* out = faces + (long long)A * Q * P.eN + e;
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    Real v = lpost[ip];
+    if constexpr (cv(A, ip) != 0) {
+      if (low || high) {
+        const bool in = (low && cv(A, ip) > 0) || (high && cv(A, ip) < 0);
+        v = in ? Real(0) : v + lpost[OPP[ip]];
+      }
+    }
+    out[(long long)ip * P.eN] = v;
+  });
 }
 
-// Collision cotangent of population ip at the face of axis A with left cell
-// slot s (ghost-inclusive coordinates mf), before bounce-back.
-template <int A, int ip>
-__device__ __forceinline__ double face_base(const KParams& P, const double* lam, const double* faces,
-                                            const int mf[3], long long ef) {
-  const double* fa = faces + (long long)A * 4 * P.eN;
-  double v = fa[ef] + fa[P.eN + ef] * CX[ip] + fa[2 * P.eN + ef] * CY[ip] + fa[3 * P.eN + ef] * CZ[ip];
-  if constexpr (cv(A, ip) != 0) {
-    int mR[3] = {mf[0], mf[1], mf[2]};
-    mR[A] += 1;
-    const double updL = P.tab[A][mf[A] * 9 + 8], updR = P.tab[A][(mf[A] + 1) * 9 + 8];
-    const double lL = lam_at(P, lam, ip, mf[0], mf[1], mf[2]);
-    const double lR = lam_at(P, lam, ip, mR[0], mR[1], mR[2]);
-    v += P.fw1 * cv(A, ip) * (updR * lR - updL * lL);
-  }
-  return v;
+// Face-slot index of a periodic axis' ghost position in the weight table
+// (the table rows of ghost positions are only used for wall axes).
+__device__ __forceinline__ int table_slot(const KParams& P, int a, int m) {
+  if (!P.periodic[a]) return m;
+  return m == 0 ? P.n[a] : (m == P.n[a] + 1 ? 1 : m);
 }
 
-// ... after bounce-back at wall faces: incoming populations were replaced by
-// their opposites, so their cotangent moves to the opposite.
-template <int A, int ip>
-__device__ __forceinline__ double face_lambda(const KParams& P, const double* lam, const double* faces,
-                                              const int mf[3], long long ef) {
-  const double base = face_base<A, ip>(P, lam, faces, mf, ef);
-  if constexpr (cv(A, ip) == 0) {
-    return base;
-  } else {
-    if (P.periodic[A]) return base;
-    const bool low = mf[A] == 0, high = mf[A] == P.n[A];
-    if (!low && !high) return base;
-    const bool incoming = (low && cv(A, ip) > 0) || (high && cv(A, ip) < 0);
-    if (incoming) return 0.0;
-    return base + face_base<A, OPP[ip]>(P, lam, faces, mf, ef);
-  }
-}
-
-// S3^T at ghost-inclusive position t for population ip: sum over the faces
-// of axis A whose reconstruction stencil contains t, times the stencil weight.
-template <int A, int ip>
-__device__ __forceinline__ double gather_axis(const KParams& P, const double* lam, const double* faces, const int t[3]) {
+// S3^T at ghost-inclusive position t (linear index et) for population ip: the
+// sum over the faces of axis A whose reconstruction stencil contains t of
+// the face cotangent times the stencil weight.  Face arrays hold periodic
+// ghost copies (k_face_ghosts) and zeros at the ghost rows of wall axes, so
+// every stencil face of an interior position is a plain offset from et.  A
+// wall ghost position (GHOST) can reach outside the array and is checked.
+template <class Real, int A, int ip, bool GHOST>
+__device__ __forceinline__ Real gather_axis(const KParams& P, const Real* faces, const int t[3], long long et) {
   constexpr int c[3] = {CX[ip], CY[ip], CZ[ip]};
   constexpr int B1 = (A + 1) % 3, B2 = (A + 2) % 3;
-  double acc = 0;
-  // normal: t is the left cell of face mL = t_A, or the right cell of mL = t_A - 1
+  const long long st[3] = {1, P.e[0], (long long)P.e[0] * P.e[1]};
+  const Real* fa = faces + ((long long)A * Q + ip) * P.eN + et;
+  Real acc = 0;
+  // normal: t is the left cell of the face in slot t_A, the right cell of slot t_A - 1
 #pragma unroll
   for (int side = 0; side < 2; side++) {
-    const int mLraw = t[A] - side;
-    const int sl = face_slot(P, A, mLraw);
-    if (sl < 0) continue;
-    double wn;
+    const int sl = t[A] - side;
+    if (GHOST && (sl < 0 || sl > P.n[A])) continue;
+    Real wn;
     if constexpr (ip == 0) {
-      wn = 0.5;
+      wn = Real(0.5);
     } else {
-      const double wR = P.tab[A][sl * 9 + c[A] + 1];  // fraction toward the right cell
-      wn = side == 0 ? 1.0 - wR : wR;
+      const Real wR = tabv<Real>(P, A, table_slot(P, A, sl) * 9 + c[A] + 1);  // toward the right cell
+      wn = side == 0 ? Real(1) - wR : wR;
     }
 #pragma unroll
     for (int s1 = 0; s1 < 2; s1++) {
-      if (c[B1] == 0 && s1 == 1) continue;
-      if (ip == 0 && s1 == 1) continue;
-      const int f1 = tangential_slot(P, B1, t[B1] + s1 * c[B1]);
-      if (f1 < 0) continue;
-      double w1 = 1.0;
+      if (s1 == 1 && (c[B1] == 0 || ip == 0)) continue;
+      const int f1 = t[B1] + s1 * c[B1];
+      if (GHOST && (f1 < 0 || f1 > P.n[B1] + 1)) continue;
+      Real w1 = 1;
       if (c[B1] != 0 && ip != 0) {
-        const double m = P.tab[B1][f1 * 9 + (c[B1] < 0 ? 6 : 7)];
-        w1 = s1 ? m : 1.0 - m;
+        const Real m = tabv<Real>(P, B1, table_slot(P, B1, f1) * 9 + (c[B1] < 0 ? 6 : 7));
+        w1 = s1 ? m : Real(1) - m;
       }
 #pragma unroll
       for (int s2 = 0; s2 < 2; s2++) {
-        if (c[B2] == 0 && s2 == 1) continue;
-        if (ip == 0 && s2 == 1) continue;
-        const int f2 = tangential_slot(P, B2, t[B2] + s2 * c[B2]);
-        if (f2 < 0) continue;
-        double w2 = 1.0;
+        if (s2 == 1 && (c[B2] == 0 || ip == 0)) continue;
+        const int f2 = t[B2] + s2 * c[B2];
+        if (GHOST && (f2 < 0 || f2 > P.n[B2] + 1)) continue;
+        Real w2 = 1;
         if (c[B2] != 0 && ip != 0) {
-          const double m = P.tab[B2][f2 * 9 + (c[B2] < 0 ? 6 : 7)];
-          w2 = s2 ? m : 1.0 - m;
+          const Real m = tabv<Real>(P, B2, table_slot(P, B2, f2) * 9 + (c[B2] < 0 ? 6 : 7));
+          w2 = s2 ? m : Real(1) - m;
         }
-        int mf[3];
-        mf[A] = sl;
-        mf[B1] = f1;
-        mf[B2] = f2;
-        const long long ef = ext_index(P, mf[0], mf[1], mf[2]);
-        acc += wn * w1 * w2 * face_lambda<A, ip>(P, lam, faces, mf, ef);
+        const long long off = -side * st[A] + s1 * c[B1] * st[B1] + s2 * c[B2] * st[B2];
+        acc += wn * w1 * w2 * fa[off];
       }
     }
   }
   return acc;
 }
 
-template <int ip>
-__device__ __forceinline__ double gather(const KParams& P, const double* lam, const double* faces, const int t[3]) {
-  return gather_axis<0, ip>(P, lam, faces, t) + gather_axis<1, ip>(P, lam, faces, t) +
-         gather_axis<2, ip>(P, lam, faces, t);
+template <class Real, int ip, bool GHOST>
+__device__ __forceinline__ Real gather(const KParams& P, const Real* faces, const int t[3]) {
+  const long long et = ext_index(P, t[0], t[1], t[2]);
+  return gather_axis<Real, 0, ip, GHOST>(P, faces, t, et) + gather_axis<Real, 1, ip, GHOST>(P, faces, t, et) +
+         gather_axis<Real, 2, ip, GHOST>(P, faces, t, et);
+}
+
+// Periodic ghost copies of the face arrays along axis b (slot 0 <- n,
+// n+1 <- 1), for all three face axes and 19 populations; axes in the order
+// x, y, z so that edges and corners are consistent.
+template <class Real>
+__global__ void __launch_bounds__(256) k_face_ghosts(KParams P, Real* __restrict__ faces, int b) {
+  const int b1 = (b + 1) % 3, b2 = (b + 2) % 3;
+  const long long plane = (long long)P.e[b1] * P.e[b2];
+  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= plane * 3 * Q) return;
+  const long long arr = id / plane, pos = id % plane;
+  int m[3];
+  m[b1] = (int)(pos % P.e[b1]);
+  m[b2] = (int)(pos / P.e[b1]);
+  Real* fa = faces + arr * P.eN;
+  const int n = P.n[b];
+  m[b] = n;
+  const Real vn = fa[ext_index(P, m[0], m[1], m[2])];
+  m[b] = 1;
+  const Real v1 = fa[ext_index(P, m[0], m[1], m[2])];
+  m[b] = 0;
+  fa[ext_index(P, m[0], m[1], m[2])] = vn;
+  m[b] = n + 1;
+  fa[ext_index(P, m[0], m[1], m[2])] = v1;
 }
 
 // ---- kernel 3: one thread per interior cell ----
 template <class Real, class Store>
 __global__ void __launch_bounds__(256) k_adj_cells(KParams P, const Store* __restrict__ f,
-                                                   const double* __restrict__ lam, const double* __restrict__ faces,
+                                                   const double* __restrict__ lam, const Real* __restrict__ faces,
                                                    double* __restrict__ lam_prev, double* __restrict__ grad,
                                                    int wall_axis) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -313,23 +266,23 @@ __global__ void __launch_bounds__(256) k_adj_cells(KParams P, const Store* __res
   Real lfb[Q];
   static_for<Q>([&](auto I) {
     constexpr int ip = decltype(I)::value;
-    double v = 4.0 / 3.0 * lam[ip * P.N + c] + gather<ip>(P, lam, faces, t);
+    Real v = Real(4.0 / 3.0) * (Real)lam[ip * P.N + c] + gather<Real, ip, false>(P, faces, t);
     if (wall_axis >= 0) {
       // S2^T: the wall ghost g = q f(1) + (1-q) f(2) (and mirrored at the top)
       const int W = wall_axis, n = P.n[W];
       int g[3] = {t[0], t[1], t[2]};
       if (t[W] == 1 || t[W] == 2) {
         g[W] = 0;
-        const double q = P.qlo[W];
-        v += (t[W] == 1 ? q : 1.0 - q) * gather<ip>(P, lam, faces, g);
+        const Real q = (Real)P.qlo[W];
+        v += (t[W] == 1 ? q : Real(1) - q) * gather<Real, ip, true>(P, faces, g);
       }
       if (t[W] == n || t[W] == n - 1) {
         g[W] = n + 1;
-        const double q = P.qhi[W];
-        v += (t[W] == n ? q : 1.0 - q) * gather<ip>(P, lam, faces, g);
+        const Real q = (Real)P.qhi[W];
+        v += (t[W] == n ? q : Real(1) - q) * gather<Real, ip, true>(P, faces, g);
       }
     }
-    lfb[ip] = (Real)v;
+    lfb[ip] = v;
   });
   // S1^T
   Real ff[Q];
@@ -370,16 +323,33 @@ __global__ void __launch_bounds__(256) k_adj_cells(KParams P, const Store* __res
 template <class Real, class Store>
 hipError_t run_adjoint(Ctx& cx, const void* f, const double* lam_next, double* lam_prev, double* grad, int wall) {
   const KParams& P = cx.kp;
-  hipError_t e = launch_fbar_ghosts(cx, f, cx.fbx);
+  Real* faces = static_cast<Real*>(cx.adj_faces);
+  bool handled = false;
+  hipError_t e = launch_adjoint_faces_fused(cx, f, lam_next, faces, &handled);
   if (e != hipSuccess) return e;
-  const unsigned nbe = (unsigned)((P.eN + 255) / 256);
-  const Real* fbx = static_cast<const Real*>(cx.fbx);
-  hipLaunchKernelGGL((k_adj_faces<Real, 0>), dim3(nbe), dim3(256), 0, cx.stream, P, fbx, lam_next, cx.adj_faces);
-  hipLaunchKernelGGL((k_adj_faces<Real, 1>), dim3(nbe), dim3(256), 0, cx.stream, P, fbx, lam_next, cx.adj_faces);
-  hipLaunchKernelGGL((k_adj_faces<Real, 2>), dim3(nbe), dim3(256), 0, cx.stream, P, fbx, lam_next, cx.adj_faces);
+  if (!handled) {
+    if (!cx.fbx) {
+      e = cx.alloc(&cx.fbx, (size_t)P.eN * Q * sizeof(Real));
+      if (e != hipSuccess) return e;
+    }
+    e = launch_fbar_ghosts(cx, f, cx.fbx);
+    if (e != hipSuccess) return e;
+    const unsigned nbe = (unsigned)((P.eN + 255) / 256);
+    const Real* fbx = static_cast<const Real*>(cx.fbx);
+    hipLaunchKernelGGL((k_adj_faces<Real, 0>), dim3(nbe), dim3(256), 0, cx.stream, P, fbx, lam_next, faces);
+    hipLaunchKernelGGL((k_adj_faces<Real, 1>), dim3(nbe), dim3(256), 0, cx.stream, P, fbx, lam_next, faces);
+    hipLaunchKernelGGL((k_adj_faces<Real, 2>), dim3(nbe), dim3(256), 0, cx.stream, P, fbx, lam_next, faces);
+  }
+  for (int b = 0; b < 3; b++) {
+    if (!P.periodic[b]) continue;
+    const long long cnt = (long long)P.e[(b + 1) % 3] * P.e[(b + 2) % 3] * 3 * Q;
+    hipLaunchKernelGGL((k_face_ghosts<Real>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, cx.stream, P, faces,
+                       b);
+  }
   const unsigned nbc = (unsigned)((P.N + 255) / 256);
   hipLaunchKernelGGL((k_adj_cells<Real, Store>), dim3(nbc), dim3(256), 0, cx.stream, P,
-                     static_cast<const Store*>(f), lam_next, cx.adj_faces, lam_prev, grad, wall);
+                     static_cast<const Store*>(f), lam_next, static_cast<const Real*>(cx.adj_faces), lam_prev, grad,
+                     wall);
   return hipGetLastError();
 }
 
@@ -396,14 +366,11 @@ hipError_t launch_adjoint(Ctx& c, const void* f, const double* lam_next, double*
     }
   if (walls > 1) return hipErrorNotSupported;
   const size_t rb = c.precision == 0 ? 8 : 4;
-  if (!c.fbx) {
-    hipError_t e = c.alloc(&c.fbx, (size_t)c.kp.eN * Q * rb);
-    if (e != hipSuccess) return e;
-  }
   if (!c.adj_faces) {
-    hipError_t e = c.alloc((void**)&c.adj_faces, (size_t)c.kp.eN * 12 * sizeof(double));
+    const size_t bytes = (size_t)c.kp.eN * 3 * Q * rb;
+    hipError_t e = c.alloc(&c.adj_faces, bytes);
     if (e != hipSuccess) return e;
-    e = hipMemsetAsync(c.adj_faces, 0, (size_t)c.kp.eN * 12 * sizeof(double), c.stream);
+    e = hipMemsetAsync(c.adj_faces, 0, bytes, c.stream);
     if (e != hipSuccess) return e;
   }
   switch (c.precision) {

@@ -367,6 +367,7 @@ void dugks_free(dugks_ctx* ctx) {
   if (c.stream) hipStreamSynchronize(c.stream);
   for (int b = 0; b < 2; b++) hipFree(c.f[b]);
   hipFree(c.fbx);
+  hipFree(c.adj_faces);
   for (int a = 0; a < 3; a++) {
     hipFree(c.tab[a]);
     hipFree(c.tabf[a]);
@@ -696,15 +697,13 @@ int dugks_forward_adjoint(dugks_ctx* ctx, int istep0, int T, int loss_kind, int 
   const size_t sb = c.store_bytes;
   // checkpoint store (state at the start of each segment) + segment states
   std::vector<void*> cps(ncp, nullptr), seg(interval, nullptr);
-  double *lam = nullptr, *lam2 = nullptr, *g = nullptr, *dJ = nullptr;
+  double *lam = nullptr, *lam2 = nullptr;
   double bytes = 0;
   auto cleanup = [&]() {
     for (void* p : cps) hipFree(p);
     for (void* p : seg) hipFree(p);
     hipFree(lam);
     hipFree(lam2);
-    hipFree(g);
-    hipFree(dJ);
   };
   auto chk = [&](hipError_t e, const char* what) {
     if (e != hipSuccess) {
@@ -725,11 +724,12 @@ int dugks_forward_adjoint(dugks_ctx* ctx, int istep0, int T, int loss_kind, int 
   }
   if ((rc = chk(hipMalloc((void**)&lam, nf * sizeof(double)), "lambda alloc"))) return rc;
   if ((rc = chk(hipMalloc((void**)&lam2, nf * sizeof(double)), "lambda alloc"))) return rc;
-  if ((rc = chk(hipMalloc((void**)&dJ, nf * sizeof(double)), "lambda alloc"))) return rc;
-  bytes += 3.0 * nf * sizeof(double);
+  bytes += 2.0 * nf * sizeof(double);
+  // dval[0]: sum of the loss over the forward sweep (accumulated on the
+  // device, read once); dval[1]: discarded values of the reverse sweep
   double* dval = c.red + reduction_scratch_doubles() + 9;
+  if ((rc = chk(hipMemsetAsync(dval, 0, 2 * sizeof(double), c.stream), "memset"))) return rc;
   // forward sweep: checkpoints and the loss
-  double Jsum = 0;
   for (int n = 0; n < T; n++) {
     if (n % interval == 0)
       if ((rc = chk(hipMemcpyAsync(cps[n / interval], c.f[c.cur], nf * sb, hipMemcpyDeviceToDevice, c.stream),
@@ -739,13 +739,12 @@ int dugks_forward_adjoint(dugks_ctx* ctx, int istep0, int T, int loss_kind, int 
       cleanup();
       return rc;
     }
-    double v = 0;
-    if ((rc = chk(launch_loss(c, c.f[c.cur], loss_kind, dval, nullptr), "loss"))) return rc;
-    if ((rc = chk(hipMemcpyAsync(&v, dval, sizeof(double), hipMemcpyDeviceToHost, c.stream), "loss copy")))
-      return rc;
-    if ((rc = chk(hipStreamSynchronize(c.stream), "sync"))) return rc;
-    Jsum += v;
+    if ((rc = chk(launch_loss(c, c.f[c.cur], loss_kind, dval, nullptr, 0, true), "loss"))) return rc;
   }
+  double Jsum = 0;
+  if ((rc = chk(hipMemcpyAsync(&Jsum, dval, sizeof(double), hipMemcpyDeviceToHost, c.stream), "loss copy")))
+    return rc;
+  if ((rc = chk(hipStreamSynchronize(c.stream), "sync"))) return rc;
   *J = Jsum / T;
   const void* fT = c.f[c.cur];  // final state; the context keeps it
   // reverse sweep
@@ -768,8 +767,7 @@ int dugks_forward_adjoint(dugks_ctx* ctx, int istep0, int T, int loss_kind, int 
     for (int n = n1 - 1; n >= n0; n--) {
       // lam currently = dJ/df^{n+1}; add the loss term at step n+1
       const void* fnext = (n + 1 < n1) ? seg[n + 1 - n0] : (n1 == T ? fT : cps[q + 1]);
-      if ((rc = chk(launch_loss(c, fnext, loss_kind, dval, dJ), "loss grad"))) return rc;
-      if ((rc = chk(launch_axpy(c, invT, dJ, lam, (long long)nf), "axpy"))) return rc;
+      if ((rc = chk(launch_loss(c, fnext, loss_kind, dval + 1, lam, invT), "loss grad"))) return rc;
       // lam2 = (df^{n+1}/df^n)^T lam, about f^n = seg[n - n0]
       if ((rc = chk(launch_adjoint(c, seg[n - n0], lam, lam2, dev_grad_force), "adjoint"))) return rc;
       std::swap(lam, lam2);

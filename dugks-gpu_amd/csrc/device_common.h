@@ -174,4 +174,48 @@ __device__ __forceinline__ void centre_fbar(const Real (&f)[Q], const Real F[3],
   });
 }
 
+// Adjoint of the equilibrium-with-force term (used by the adjoint of the
+// centre stage and of the face collision).
+// d/d(rho, u, F) of sum_i lam_i (B + C T_i) feq_i(rho, u), T_i = (c_i - u).F/RT,
+// over the populations in the compile-time set Sel (Sel(ip) true).
+template <class Real, class Sel>
+__device__ __forceinline__ void feq_adjoint(const Real (&lam)[Q], Real B, Real C, Real rho, const Real u[3],
+                                            const Real F[3], Real& g_rho, Real g_u[3], Real g_F[3], Sel sel) {
+  const Real uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  const Real uF = u[0] * F[0] + u[1] * F[1] + u[2] * F[2];
+  g_rho = 0;
+  for (int a = 0; a < 3; a++) g_u[a] = g_F[a] = 0;
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    if constexpr (decltype(sel(I))::value) {
+      const Real c[3] = {Real(CX[ip]), Real(CY[ip]), Real(CZ[ip])};
+      const Real cu = c[0] * u[0] + c[1] * u[1] + c[2] * u[2];
+      const Real eu = Real(3) * cu;
+      const Real poly = Real(1) + eu + Real(0.5) * (eu * eu - Real(3) * uu);
+      const Real w = Real(wgt(ip));
+      const Real feq = w * rho * poly;
+      const Real cF = c[0] * F[0] + c[1] * F[1] + c[2] * F[2];
+      const Real T = Real(3) * (cF - uF);
+      const Real coef = B + C * T;
+      const Real l = lam[ip];
+      g_rho += l * coef * w * poly;
+      for (int a = 0; a < 3; a++) {
+        const Real dfeq = w * rho * Real(3) * (c[a] + eu * c[a] - u[a]);
+        g_u[a] += l * (coef * dfeq - C * Real(3) * F[a] * feq);
+        g_F[a] += l * C * Real(3) * (c[a] - u[a]) * feq;
+      }
+    }
+  });
+}
+
+struct AllPops {
+  template <class I>
+  __device__ constexpr std::true_type operator()(I) const { return {}; }
+};
+template <int A>
+struct Crossing {
+  template <class I>
+  __device__ constexpr std::integral_constant<bool, cv(A, I::value) != 0> operator()(I) const { return {}; }
+};
+
 }  // namespace dugks

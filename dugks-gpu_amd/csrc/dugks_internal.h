@@ -54,6 +54,9 @@ struct Ctx;
 hipError_t launch_step_twopass(Ctx& c, const void* fin, void* fout);
 hipError_t launch_fbar_ghosts(Ctx& c, const void* fin, void* fbx);
 hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled);
+// Adjoint-face pass of the fused kernel (see adjoint.hip); handled = false
+// when the fused kernel does not cover the configuration.
+hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const double* lam, void* faces, bool* handled);
 hipError_t launch_macro(Ctx& c, const void* f, double* rho, double* ux, double* uy, double* uz);
 hipError_t launch_export(Ctx& c, const void* f, double* dst);
 hipError_t launch_import(Ctx& c, const double* src, void* f);
@@ -62,7 +65,11 @@ hipError_t launch_flow_control(Ctx& c, const void* f);
 hipError_t launch_perturb_force(Ctx& c, int step);
 hipError_t launch_adjoint(Ctx& c, const void* f, const double* lam_next, double* lam_prev,
                           double* grad_force);
-hipError_t launch_loss(Ctx& c, const void* f, int kind, double* dev_value, double* dev_grad);
+// Loss value into dev_value (added to it when value_acc) and, when dev_grad
+// is set, its gradient dJ/df written into dev_grad (grad_acc == 0) or added
+// to it times grad_acc.
+hipError_t launch_loss(Ctx& c, const void* f, int kind, double* dev_value, double* dev_grad, double grad_acc = 0,
+                       bool value_acc = false);
 hipError_t launch_axpy(Ctx& c, double a, const double* x, double* y, long long n);
 
 // Library context.
@@ -78,7 +85,7 @@ struct Ctx {
   void* f[2] = {nullptr, nullptr};
   int cur = 0;                // f[cur] holds the current state
   void* fbx = nullptr;        // ghost-inclusive f-bar (two-pass kernels, adjoint)
-  double* adj_faces = nullptr; // adjoint: 4 scalars per face, 3 axes, ghost-inclusive
+  void* adj_faces = nullptr;    // adjoint: 19 face cotangents per face, 3 axes, ghost-inclusive, arithmetic type
   double* tab[3] = {nullptr, nullptr, nullptr};
   double* wd[3] = {nullptr, nullptr, nullptr};
   float* tabf[3] = {nullptr, nullptr, nullptr};

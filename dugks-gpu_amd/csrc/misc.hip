@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(kRedThreads) k_sums(KParams P, const Store* __
 
 template <int NV>
 __global__ void __launch_bounds__(kRedThreads) k_reduce_final(const double* partial, int nparts,
-                                                             double* out) {
+                                                             double* out, int accumulate) {
   double acc[NV];
   for (int q = 0; q < NV; q++) acc[q] = 0;
   for (int b = threadIdx.x; b < nparts; b += kRedThreads)
@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce_final(const double* part
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    for (int q = 0; q < NV; q++) out[q] = sh[q][0];
+    for (int q = 0; q < NV; q++) out[q] = (accumulate ? out[q] : 0.0) + sh[q][0];
 }
 
 // Constant-flow-rate controller (single thread).  famb_dev layout:
@@ -191,7 +191,8 @@ __global__ void __launch_bounds__(256) k_perturb(KParams P, int step, double xwi
 template <class Store>
 __global__ void __launch_bounds__(kRedThreads) k_loss(KParams P, const Store* __restrict__ f, int kind,
                                                       int aw, int as, double visc, double vol,
-                                                      double nplane, double* grad, double* partial) {
+                                                      double nplane, double* grad, double gacc,
+                                                      double* partial) {
   double acc[1] = {0};
   double F[3];
   ambient_d(P, F);
@@ -218,7 +219,12 @@ __global__ void __launch_bounds__(kRedThreads) k_loss(KParams P, const Store* __
         const double g = coef * (cv(0, ip) * (as == 0) + cv(1, ip) * (as == 1) + cv(2, ip) * (as == 2) +
                                  P.hdt * F[as]);
         val += g * x[ip];
-        if (grad) grad[ip * P.N + c] = g;
+        if (grad) {
+          if (gacc == 0.0)
+            grad[ip * P.N + c] = g;
+          else if (wall)
+            grad[ip * P.N + c] += gacc * g;
+        }
       });
       acc[0] += val;
     } else {
@@ -242,7 +248,11 @@ __global__ void __launch_bounds__(kRedThreads) k_loss(KParams P, const Store* __
           constexpr int ip = decltype(I)::value;
           const double cu = (CX[ip] + P.hdt * F[0]) * u[0] + (CY[ip] + P.hdt * F[1]) * u[1] +
                             (CZ[ip] + P.hdt * F[2]) * u[2];
-          grad[ip * P.N + c] = (dV / vol) * (cu - 0.5 * uu);
+          const double g = (dV / vol) * (cu - 0.5 * uu);
+          if (gacc == 0.0)
+            grad[ip * P.N + c] = g;
+          else
+            grad[ip * P.N + c] += gacc * g;
         });
       }
     }
@@ -284,12 +294,12 @@ hipError_t sums_t(Ctx& c, const void* f, double* out) {
   hipLaunchKernelGGL((k_sums<Store>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.stream, c.kp,
                      static_cast<const Store*>(f), c.red);
   hipLaunchKernelGGL((k_reduce_final<8>), dim3(1), dim3(kRedThreads), 0, c.stream, c.red, kRedBlocks,
-                     out);
+                     out, 0);
   return hipGetLastError();
 }
 
 template <class Store>
-hipError_t loss_t(Ctx& c, const void* f, int kind, double* value, double* grad) {
+hipError_t loss_t(Ctx& c, const void* f, int kind, double* value, double* grad, double gacc, bool vacc) {
   const Setup& s = c.s;
   const int aw = s.wall_axis();
   int as = s.stream_axis();
@@ -299,9 +309,9 @@ hipError_t loss_t(Ctx& c, const void* f, int kind, double* value, double* grad) 
   double nplane = aw >= 0 ? (double)s.ncell() / s.n[aw] : 1.0;
   hipLaunchKernelGGL((k_loss<Store>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.stream, c.kp,
                      static_cast<const Store*>(f), kind, aw < 0 ? 0 : aw, as, s.visc, vol, nplane, grad,
-                     c.red);
+                     gacc, c.red);
   hipLaunchKernelGGL((k_reduce_final<1>), dim3(1), dim3(kRedThreads), 0, c.stream, c.red, kRedBlocks,
-                     value);
+                     value, vacc ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -348,10 +358,11 @@ hipError_t launch_perturb_force(Ctx& c, int step) {
   return hipGetLastError();
 }
 
-hipError_t launch_loss(Ctx& c, const void* f, int kind, double* dev_value, double* dev_grad) {
-  if (c.precision == 0) return loss_t<double>(c, f, kind, dev_value, dev_grad);
-  if (c.precision == 1) return loss_t<float>(c, f, kind, dev_value, dev_grad);
-  return loss_t<__half>(c, f, kind, dev_value, dev_grad);
+hipError_t launch_loss(Ctx& c, const void* f, int kind, double* dev_value, double* dev_grad, double grad_acc,
+                       bool value_acc) {
+  if (c.precision == 0) return loss_t<double>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
+  if (c.precision == 1) return loss_t<float>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
+  return loss_t<__half>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
 }
 
 hipError_t launch_axpy(Ctx& c, double a, const double* x, double* y, long long n) {

@@ -205,18 +205,77 @@ struct Face {
       out[k] = fw1 * cross[k] + (P0 + Real(3) * fw0 * cF) * feq;
     });
   }
+  // Transpose of collide (after bounce) and bounce: given the cotangent mu
+  // of the ten outputs, the cotangent of the 19 reconstructed values before
+  // bounce-back.  d out_q / d fbar_p = fw1 delta_qp + (d/d rho + c_p.d/d j)
+  // of the equilibrium term, so lam_p = fw1 mu_p + A + B.c_p; a population
+  // that entered through a wall took its opposite's value, so its cotangent
+  // moves to that opposite.
+  __device__ __forceinline__ void adjoint(const Real F[3], Real hdt, Real fw0, Real fw1, Real fw2,
+                                          const Real (&mu)[10], bool low_wall, bool high_wall, Real (&lam)[Q]) {
+    static_for<10>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      constexpr int ip = dirs<A>(k);
+      const Real v = cross[k];
+      rho += v;
+      if constexpr (CX[ip] != 0) jx += CX[ip] > 0 ? v : -v;
+      if constexpr (CY[ip] != 0) jy += CY[ip] > 0 ? v : -v;
+      if constexpr (CZ[ip] != 0) jz += CZ[ip] > 0 ? v : -v;
+    });
+    const Real inv = rcp_real(rho);
+    const Real hh = Real(0.5) * hdt;
+    const Real ut[3] = {jx * inv, jy * inv, jz * inv};
+    const Real u[3] = {ut[0] + hh * F[0], ut[1] + hh * F[1], ut[2] + hh * F[2]};
+    Real lq[Q];
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      constexpr int k = cross_index<A>(ip);
+      if constexpr (k >= 0) lq[ip] = mu[k]; else lq[ip] = Real(0);
+    });
+    Real g_rho, g_u[3], g_F[3];
+    feq_adjoint<Real>(lq, fw2, fw0, rho, u, F, g_rho, g_u, g_F, Crossing<A>{});
+    const Real Ac = g_rho - (g_u[0] * ut[0] + g_u[1] * ut[1] + g_u[2] * ut[2]) * inv;
+    const Real Bx = g_u[0] * inv, By = g_u[1] * inv, Bz = g_u[2] * inv;
+    Real post[Q];
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      post[ip] = fw1 * lq[ip] + Ac + Bx * Real(CX[ip]) + By * Real(CY[ip]) + Bz * Real(CZ[ip]);
+    });
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      Real v = post[ip];
+      if constexpr (cv(A, ip) != 0) {
+        const bool in = (low_wall && cv(A, ip) > 0) || (high_wall && cv(A, ip) < 0);
+        const bool opp_in = (low_wall && cv(A, ip) < 0) || (high_wall && cv(A, ip) > 0);
+        if (in) v = Real(0);
+        if (opp_in) v += post[OPP[ip]];
+      }
+      lam[ip] = v;
+    });
+  }
 };
 
-template <class Real, class Store, int TX, int TY>
+// Inputs and outputs of the adjoint-face mode of the fused kernel.
+struct AdjFaceIO {
+  const double* lam;  // lam' = dJ/df(n+1), SoA [19][nz][ny][nx]
+  void* faces;        // out: [3][19][ghost-inclusive face slot], arithmetic type
+};
+
+// ADJ = false: one DUGKS step fin -> fout.
+// ADJ = true: the adjoint-face pass of the step taken from fin: every face is
+// reconstructed and its collision linearised as in the step, and the
+// cotangents of its 19 reconstructed values, driven by lam' (adj.lam), are
+// written to adj.faces; nothing else is written.
+template <class Real, class Store, int TX, int TY, bool ADJ = false>
 __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const Store* __restrict__ fin,
                                                           Store* __restrict__ fout, int TZ,
-                                                          double* __restrict__ partial) {
+                                                          double* __restrict__ partial, AdjFaceIO adj) {
   using T = Tile<TX, TY>;
   constexpr int AC = T::AC;
   __shared__ Real A[Q][TY + 2][AC];        // Z(p): tangential z-lerp of f-bar(p)
   __shared__ Real An[Q][TY + 2][AC];       // Zn(p+1/2): normal z-lerp between p and p+1
-  __shared__ Real Cx[10][TY][TX + 1];      // crossing populations of -x faces
-  __shared__ Real Dy[10][TY + 1][TX];      // crossing populations of -y faces
+  __shared__ Real Cx[ADJ ? 1 : 10][TY][TX + 1];  // crossing populations of -x faces
+  __shared__ Real Dy[ADJ ? 1 : 10][TY + 1][TX];  // crossing populations of -y faces
   __shared__ Real WX[6][TX + 2];           // weights per tile column
   __shared__ Real WY[6][TY + 2];           // weights per tile row
 
@@ -314,8 +373,7 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
   };
   // +z face of an interior column from Zn in A: bilinear over the upwind
   // tangential neighbours, then collision; out = crossing populations
-  auto zface = [&](Real (&out)[10]) {
-    Face<Real, 2> fc;
+  auto zface_rec = [&](Face<Real, 2>& fc) {
     static_for<Q>([&](auto I) {
       constexpr int ip = decltype(I)::value;
       constexpr int cx = CX[ip], cy = CY[ip];
@@ -334,8 +392,25 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
       }
       fc.template put<ip>(v);
     });
+  };
+  auto zface = [&](Real (&out)[10]) {
+    Face<Real, 2> fc;
+    zface_rec(fc);
     fc.collide(Famb, hdt, fw0, fw1, fw2, out);
   };
+  // adjoint-face mode: lam' of population ip at an interior cell (0 for a
+  // ghost, cell < 0) and the store of a face's cotangents
+  auto lam_at = [&](int ip, long long cell) -> Real {
+    return cell < 0 ? Real(0) : (Real)adj.lam[(long long)ip * P.N + cell];
+  };
+  auto store_face = [&](int a, long long slot, const Real (&lam)[Q]) {
+    Real* out = static_cast<Real*>(adj.faces) + (long long)a * Q * P.eN + slot;
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      out[(long long)ip * P.eN] = lam[ip];
+    });
+  };
+  const long long e0 = P.e[0], e1 = P.e[1];
 
   // ---- pre-loop: f-bar at z0-1 and z0, and the -z face of plane z0 ----
   {
@@ -350,7 +425,9 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
     }
     store_zn(fbn, fbc);
     __syncthreads();
-    if (interior) zface(zm);
+    if constexpr (!ADJ) {
+      if (interior) zface(zm);
+    }
     __syncthreads();
   }
 
@@ -375,8 +452,11 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
     store_zn(fbc, fbn);
     __syncthreads();
     // ---- phase 2: the faces of plane p (-x, -y from Z; +z from Zn) ----
-    const bool do_x = interior || (active && t >= T::NROW && c == TX && r >= 1 && r <= TY);
-    const bool do_y = (t < T::NROW && r >= 1);  // interior rows and the top halo row
+    // the forward needs the tile's right and top boundary faces too; the
+    // adjoint-face pass leaves them to the next tile except at a wall
+    const bool xhalo = active && t >= T::NROW && c == TX && r >= 1 && r <= TY;
+    const bool do_x = interior || (xhalo && (!ADJ || (!P.periodic[0] && gi == nx)));
+    const bool do_y = t < T::NROW && r >= 1 && (!ADJ || r <= TY || (!P.periodic[1] && gj == ny));
     if (do_x) {
       // face between x-columns gi-1 (left, ghost-inclusive index mi-1) and gi
       Face<Real, 0> fc;
@@ -396,13 +476,30 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
           fc.template put<ip>(v);
         }
       });
-      if (!P.periodic[0]) fc.bounce(gi == 0, gi == nx);
-      Real out[10];
-      fc.collide(Famb, hdt, fw0, fw1, fw2, out);
-      static_for<10>([&](auto K) {
-        constexpr int k = decltype(K)::value;
-        Cx[k][r - 1][c] = out[k];
-      });
+      const bool lw = !P.periodic[0] && gi == 0, hw = !P.periodic[0] && gi == nx;
+      if (!P.periodic[0]) fc.bounce(lw, hw);
+      if constexpr (ADJ) {
+        const long long rowc = ((long long)p * ny + gj) * nx;
+        const long long cl = gi > 0 ? rowc + gi - 1 : (P.periodic[0] ? rowc + nx - 1 : -1);
+        const long long cr = gi < nx ? rowc + gi : -1;
+        const Real uL = WX[5][c], uR = WX[5][c + 1];
+        Real mu[10], lam[Q];
+        static_for<10>([&](auto K) {
+          constexpr int k = decltype(K)::value;
+          constexpr int ip = XD[k];
+          mu[k] = Real(CX[ip]) * (uR * lam_at(ip, cr) - uL * lam_at(ip, cl));
+        });
+        fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, lw, hw, lam);
+        const int slot = (gi == 0 && P.periodic[0]) ? nx : gi;
+        store_face(0, ((long long)(p + 1) * e1 + (gj + 1)) * e0 + slot, lam);
+      } else {
+        Real out[10];
+        fc.collide(Famb, hdt, fw0, fw1, fw2, out);
+        static_for<10>([&](auto K) {
+          constexpr int k = decltype(K)::value;
+          Cx[k][r - 1][c] = out[k];
+        });
+      }
     }
     if (do_y) {
       // face between y-rows gj-1 (below) and gj
@@ -423,51 +520,88 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
           fc.template put<ip>(lerp_(lo, hi, w));
         }
       });
-      if (!P.periodic[1]) fc.bounce(gj == 0, gj == ny);
-      Real out[10];
-      fc.collide(Famb, hdt, fw0, fw1, fw2, out);
-      static_for<10>([&](auto K) {
-        constexpr int k = decltype(K)::value;
-        Dy[k][r - 1][c] = out[k];
-      });
+      const bool lw = !P.periodic[1] && gj == 0, hw = !P.periodic[1] && gj == ny;
+      if (!P.periodic[1]) fc.bounce(lw, hw);
+      if constexpr (ADJ) {
+        const long long pl = (long long)p * ny;
+        const long long cl = gj > 0 ? (pl + gj - 1) * nx + gi : (P.periodic[1] ? (pl + ny - 1) * nx + gi : -1);
+        const long long cr = gj < ny ? (pl + gj) * nx + gi : -1;
+        const Real uL = WY[5][r - 1], uR = WY[5][r];
+        Real mu[10], lam[Q];
+        static_for<10>([&](auto K) {
+          constexpr int k = decltype(K)::value;
+          constexpr int ip = YD[k];
+          mu[k] = Real(CY[ip]) * (uR * lam_at(ip, cr) - uL * lam_at(ip, cl));
+        });
+        fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, lw, hw, lam);
+        const int slot = (gj == 0 && P.periodic[1]) ? ny : gj;
+        store_face(1, ((long long)(p + 1) * e1 + slot) * e0 + (gi + 1), lam);
+      } else {
+        Real out[10];
+        fc.collide(Famb, hdt, fw0, fw1, fw2, out);
+        static_for<10>([&](auto K) {
+          constexpr int k = decltype(K)::value;
+          Dy[k][r - 1][c] = out[k];
+        });
+      }
     }
     Real zp[10];
-    if (interior) zface(zp);
+    if constexpr (ADJ) {
+      if (interior) {
+        // +z face of the cell: between planes p and p+1 (z is periodic)
+        Face<Real, 2> fc;
+        zface_rec(fc);
+        const long long cl = ((long long)p * ny + gj) * nx + gi;
+        const long long cr = ((long long)((p + 1) % nz) * ny + gj) * nx + gi;
+        Real mu[10], lam[Q];
+        static_for<10>([&](auto K) {
+          constexpr int k = decltype(K)::value;
+          constexpr int ip = ZD[k];
+          mu[k] = Real(CZ[ip]) * updz * (lam_at(ip, cr) - lam_at(ip, cl));
+        });
+        fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, false, false, lam);
+        store_face(2, ((long long)(p + 1) * e1 + (gj + 1)) * e0 + (gi + 1), lam);
+      }
+    } else {
+      if (interior) zface(zp);
+    }
     __syncthreads();
     // ---- phase 3: update of plane p ----
-    if (interior) {
-      const unsigned cell = ((unsigned)p * ny + gj) * nx + gi;
-      const unsigned vb = cell * (unsigned)sizeof(Store);
-      const Real updx = WX[5][c + 1], updy = WY[5][r];
-      // f~(p+1) = 4/3 f-bar+ - 1/3 f~ - sum_a (dt/dx_a) c_a (F_a+ - F_a-)
-      Real fnew[Q];
-      static_for<Q>([&](auto I) {
-        constexpr int ip = decltype(I)::value;
-        const Real fo = ld_state<Real, Store>(bin, vb, ip * Nb, P, ip);
-        Real v = fbc[ip] * Real(4.0 / 3.0) - fo * Real(1.0 / 3.0);
-        constexpr int kx = cross_index<0>(ip), ky = cross_index<1>(ip), kz = cross_index<2>(ip);
-        if constexpr (kx >= 0) v -= Real(CX[ip]) * updx * (Cx[kx][r - 1][c + 1] - Cx[kx][r - 1][c]);
-        if constexpr (ky >= 0) v -= Real(CY[ip]) * updy * (Dy[ky][r][c] - Dy[ky][r - 1][c]);
-        if constexpr (kz >= 0) {
-          v -= Real(CZ[ip]) * updz * (zp[kz] - zm[kz]);
-          zm[kz] = zp[kz];
-        }
-        st_state<Real, Store>(bout, vb, ip * Nb, v, P, ip);
-        fnew[ip] = v;
-      });
-      if (partial) {
-        // bulk velocity of the new state for the flow-rate controller
-        // (per-cell moments in the arithmetic type, the sum over cells in FP64)
-        Real rr = 0, jj[3] = {0, 0, 0};
+    if constexpr (!ADJ) {
+      if (interior) {
+        const unsigned cell = ((unsigned)p * ny + gj) * nx + gi;
+        const unsigned vb = cell * (unsigned)sizeof(Store);
+        const Real updx = WX[5][c + 1], updy = WY[5][r];
+        // f~(p+1) = 4/3 f-bar+ - 1/3 f~ - sum_a (dt/dx_a) c_a (F_a+ - F_a-)
+        Real fnew[Q];
         static_for<Q>([&](auto I) {
           constexpr int ip = decltype(I)::value;
-          rr += fnew[ip];
-          if constexpr (CX[ip] != 0) jj[0] += CX[ip] > 0 ? fnew[ip] : -fnew[ip];
-          if constexpr (CY[ip] != 0) jj[1] += CY[ip] > 0 ? fnew[ip] : -fnew[ip];
-          if constexpr (CZ[ip] != 0) jj[2] += CZ[ip] > 0 ? fnew[ip] : -fnew[ip];
+          const Real fo = ld_state<Real, Store>(bin, vb, ip * Nb, P, ip);
+          Real v = fbc[ip] * Real(4.0 / 3.0) - fo * Real(1.0 / 3.0);
+          constexpr int kx = cross_index<0>(ip), ky = cross_index<1>(ip), kz = cross_index<2>(ip);
+          if constexpr (kx >= 0) v -= Real(CX[ip]) * updx * (Cx[kx][r - 1][c + 1] - Cx[kx][r - 1][c]);
+          if constexpr (ky >= 0) v -= Real(CY[ip]) * updy * (Dy[ky][r][c] - Dy[ky][r - 1][c]);
+          if constexpr (kz >= 0) {
+            v -= Real(CZ[ip]) * updz * (zp[kz] - zm[kz]);
+            zm[kz] = zp[kz];
+          }
+          st_state<Real, Store>(bout, vb, ip * Nb, v, P, ip);
+          fnew[ip] = v;
         });
-        const Real js = as == 0 ? jj[0] : (as == 1 ? jj[1] : jj[2]);
-        flow += (double)(js * rcp_real(rr));  // u along the flow axis; see k_flow_control
+        if (partial) {
+          // bulk velocity of the new state for the flow-rate controller
+          // (per-cell moments in the arithmetic type, the sum over cells in FP64)
+          Real rr = 0, jj[3] = {0, 0, 0};
+          static_for<Q>([&](auto I) {
+            constexpr int ip = decltype(I)::value;
+            rr += fnew[ip];
+            if constexpr (CX[ip] != 0) jj[0] += CX[ip] > 0 ? fnew[ip] : -fnew[ip];
+            if constexpr (CY[ip] != 0) jj[1] += CY[ip] > 0 ? fnew[ip] : -fnew[ip];
+            if constexpr (CZ[ip] != 0) jj[2] += CZ[ip] > 0 ? fnew[ip] : -fnew[ip];
+          });
+          const Real js = as == 0 ? jj[0] : (as == 1 ? jj[1] : jj[2]);
+          flow += (double)(js * rcp_real(rr));  // u along the flow axis; see k_flow_control
+        }
       }
     }
     if (active) {
@@ -484,7 +618,7 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
     // and the next phase 2 writes Cx/Dy only after the next phase-1 barrier
   }
 
-  if (partial) {
+  if (!ADJ && partial) {
     // fixed-order block reduction of the flow sums
     double* red = reinterpret_cast<double*>(&A[0][0][0]);
     // every column of the tile has the same z width (z is uniform here)
@@ -539,7 +673,7 @@ hipError_t run_fused(Ctx& cx, const void* fin, void* fout) {
   }
   if (cx.timing) hipEventRecord(cx.tev[0], cx.stream);
   hipLaunchKernelGGL((k_fused<Real, Store, TX, TY>), grid, dim3(T::NT), 0, cx.stream, P,
-                     static_cast<const Store*>(fin), static_cast<Store*>(fout), TZ, partial);
+                     static_cast<const Store*>(fin), static_cast<Store*>(fout), TZ, partial, AdjFaceIO{});
   if (cx.timing) hipEventRecord(cx.tev[1], cx.stream);
   if (flow) {
     const Setup& s = cx.s;
@@ -548,6 +682,18 @@ hipError_t run_fused(Ctx& cx, const void* fin, void* fout) {
                        s.stream_axis(), vol, s.flow_rate, s.flow_gain, s.dt);
   }
   cx.launches_per_step = flow ? 2 : 1;
+  return hipGetLastError();
+}
+
+template <class Real, class Store, int TX, int TY>
+hipError_t run_adj_faces(Ctx& cx, const void* f, const double* lam, void* faces) {
+  const KParams& P = cx.kp;
+  using T = Tile<TX, TY>;
+  const int TZ = cx.fused_tz > 0 ? cx.fused_tz : 32;
+  dim3 grid(P.n[0] / TX, P.n[1] / TY, (P.n[2] + TZ - 1) / TZ);
+  hipLaunchKernelGGL((k_fused<Real, Store, TX, TY, true>), grid, dim3(T::NT), 0, cx.stream, P,
+                     static_cast<const Store*>(f), static_cast<Store*>(nullptr), TZ, nullptr,
+                     AdjFaceIO{lam, faces});
   return hipGetLastError();
 }
 
@@ -584,6 +730,21 @@ hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled)
       return run_fused<float, float, 32, 8>(c, fin, fout);
     default:
       return run_fused<float, __half, 32, 8>(c, fin, fout);
+  }
+}
+
+hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const double* lam, void* faces, bool* handled) {
+  int tx = 32, ty = 4;
+  *handled = false;
+  if (c.kernel == 0 || !fused_supported(c, &tx, &ty)) return hipSuccess;
+  *handled = true;
+  switch (c.precision) {
+    case 0:
+      return run_adj_faces<double, double, 32, 4>(c, f, lam, faces);
+    case 1:
+      return run_adj_faces<float, float, 32, 8>(c, f, lam, faces);
+    default:
+      return run_adj_faces<float, __half, 32, 8>(c, f, lam, faces);
   }
 }
 

@@ -49,14 +49,22 @@ CASES = {
     "walls_x_stretched": dict(nx=16, ny=8, nz=4, periods_x=0, mesh_x="tanh:0.8", dt_rule="min",
                               ambient_fz=2e-4, xwidth=2.0, ywidth=0.8, zwidth=0.4),
     "walls_z": dict(nx=8, ny=6, nz=10, periods_z=0, ambient_fy=2e-4, xwidth=0.8, ywidth=0.6, zwidth=1.0),
+    # cases the fused adjoint-face pass covers (nx % 32 == 0, z periodic)
+    "walls_x": dict(nx=32, ny=8, nz=4, periods_x=0, ambient_fz=2e-4, xwidth=3.2, ywidth=0.8, zwidth=0.4),
+    "tiles": dict(nx=64, ny=16, nz=40, periods_y=0, mesh_y="tanh:0.85", dt_rule="min", ambient_fx=2e-4,
+                  xwidth=6.4, ywidth=2.0, zwidth=4.0),
 }
+FUSED_CASES = ("periodic", "walls_y", "walls_x", "tiles")
 
 
 @pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("kernel", [-1, KERNEL_TWOPASS], ids=["auto", "twopass"])
 @pytest.mark.parametrize("precision,tol", [(PREC_FP64, 1e-12), (PREC_FP32, 2e-5)], ids=["fp64", "fp32"])
-def test_adjoint_step_matches_oracle(rng, case, precision, tol):
+def test_adjoint_step_matches_oracle(rng, case, kernel, precision, tol):
+    if kernel == KERNEL_TWOPASS and case not in FUSED_CASES:
+        pytest.skip("the automatic choice is the two-pass path here already")
     text = configs.base(visc=0.02, **CASES[case])
-    eng = _engine(text, precision=precision)
+    eng = _engine(text, precision=precision, kernel=kernel)
     ora, famb = _oracle(eng, text)
     f0 = random_state(eng.shape, rng)
     eng.set_f(f0)
