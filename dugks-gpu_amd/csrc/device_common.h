@@ -100,6 +100,37 @@ __device__ __forceinline__ void StateBuf<__half>::put(__half v, unsigned vb, uns
   __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(v), r, (int)vb, (int)sb, 0);
 }
 
+// Raw buffer view of a plain array (< 2 GiB): per-lane 32-bit byte offset plus
+// a uniform byte offset; loads past the end return 0, stores past it are
+// dropped.
+struct RawBuf {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ RawBuf(const void* p, unsigned bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000)) {}
+  __device__ __forceinline__ double ld_f64(unsigned vb, unsigned sb) const {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vb, (int)sb, 0));
+  }
+  template <class T>
+  __device__ __forceinline__ T ld(unsigned vb, unsigned sb) const;
+  __device__ __forceinline__ void st(float v, unsigned vb, unsigned sb) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)vb, (int)sb, 0);
+  }
+  __device__ __forceinline__ void st(double v, unsigned vb, unsigned sb) const {
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)vb, (int)sb, 0);
+  }
+};
+template <>
+__device__ __forceinline__ float RawBuf::ld<float>(unsigned vb, unsigned sb) const {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vb, (int)sb, 0));
+}
+template <>
+__device__ __forceinline__ double RawBuf::ld<double>(unsigned vb, unsigned sb) const {
+  return ld_f64(vb, sb);
+}
+// A byte offset no RawBuf of this code base reaches: loads through it read 0.
+constexpr unsigned kOutOfRange = 0x80000000u;
+
 // Decode / encode one population through a StateBuf (the Codec rules).
 template <class Real, class Store>
 __device__ __forceinline__ Real ld_state(const StateBuf<Store>& b, unsigned vb, unsigned sb, const KParams& P,
@@ -176,36 +207,47 @@ __device__ __forceinline__ void centre_fbar(const Real (&f)[Q], const Real F[3],
 
 // Adjoint of the equilibrium-with-force term (used by the adjoint of the
 // centre stage and of the face collision).
-// d/d(rho, u, F) of sum_i lam_i (B + C T_i) feq_i(rho, u), T_i = (c_i - u).F/RT,
-// over the populations in the compile-time set Sel (Sel(ip) true).
+// G = sum_i lam_i (B + C T_i) feq_i(rho, u), T_i = 3 (c_i - u).F,
+// feq_i = w_i rho P_i, P_i = 1 + e_i + e_i^2/2 - 3 u.u/2, e_i = 3 c_i.u, over
+// the populations in the compile-time set Sel (Sel(ip) true).  With
+// s_i = lam_i w_i (B + C T_i):
+//   dG/drho = sum s_i P_i,
+//   dG/du   = 3 rho (sum s_i (1 + e_i) c_i - u sum s_i) - 3 C F rho sum lam_i w_i P_i,
+//   dG/dF   = 3 C rho (sum lam_i w_i P_i c_i - u sum lam_i w_i P_i),
+// so each population only adds to a handful of sums.
 template <class Real, class Sel>
 __device__ __forceinline__ void feq_adjoint(const Real (&lam)[Q], Real B, Real C, Real rho, const Real u[3],
                                             const Real F[3], Real& g_rho, Real g_u[3], Real g_F[3], Sel sel) {
   const Real uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
   const Real uF = u[0] * F[0] + u[1] * F[1] + u[2] * F[2];
-  g_rho = 0;
-  for (int a = 0; a < 3; a++) g_u[a] = g_F[a] = 0;
+  const Real base = Real(1) - Real(1.5) * uu;
+  Real S0 = 0, S1 = 0, S2 = 0, Sc[3] = {0, 0, 0}, Sp[3] = {0, 0, 0};
   static_for<Q>([&](auto I) {
     constexpr int ip = decltype(I)::value;
     if constexpr (decltype(sel(I))::value) {
-      const Real c[3] = {Real(CX[ip]), Real(CY[ip]), Real(CZ[ip])};
-      const Real cu = c[0] * u[0] + c[1] * u[1] + c[2] * u[2];
-      const Real eu = Real(3) * cu;
-      const Real poly = Real(1) + eu + Real(0.5) * (eu * eu - Real(3) * uu);
-      const Real w = Real(wgt(ip));
-      const Real feq = w * rho * poly;
-      const Real cF = c[0] * F[0] + c[1] * F[1] + c[2] * F[2];
-      const Real T = Real(3) * (cF - uF);
-      const Real coef = B + C * T;
-      const Real l = lam[ip];
-      g_rho += l * coef * w * poly;
-      for (int a = 0; a < 3; a++) {
-        const Real dfeq = w * rho * Real(3) * (c[a] + eu * c[a] - u[a]);
-        g_u[a] += l * (coef * dfeq - C * Real(3) * F[a] * feq);
-        g_F[a] += l * C * Real(3) * (c[a] - u[a]) * feq;
-      }
+      constexpr int cx = CX[ip], cy = CY[ip], cz = CZ[ip];
+      const Real cu = Real(cx) * u[0] + Real(cy) * u[1] + Real(cz) * u[2];
+      const Real cF = Real(cx) * F[0] + Real(cy) * F[1] + Real(cz) * F[2];
+      const Real e = Real(3) * cu;
+      const Real Pq = base + e * (Real(1) + Real(0.5) * e);
+      const Real lw = lam[ip] * Real(wgt(ip));
+      const Real s = lw * (B + C * Real(3) * (cF - uF));
+      const Real lp = lw * Pq;
+      const Real se = s * (Real(1) + e);
+      S0 += s;
+      S1 += s * Pq;
+      S2 += lp;
+      if constexpr (cx != 0) { Sc[0] += cx > 0 ? se : -se; Sp[0] += cx > 0 ? lp : -lp; }
+      if constexpr (cy != 0) { Sc[1] += cy > 0 ? se : -se; Sp[1] += cy > 0 ? lp : -lp; }
+      if constexpr (cz != 0) { Sc[2] += cz > 0 ? se : -se; Sp[2] += cz > 0 ? lp : -lp; }
     }
   });
+  g_rho = S1;
+  const Real r3 = Real(3) * rho;
+  for (int a = 0; a < 3; a++) {
+    g_u[a] = r3 * (Sc[a] - u[a] * S0) - r3 * C * F[a] * S2;
+    g_F[a] = r3 * C * (Sp[a] - u[a] * S2);
+  }
 }
 
 struct AllPops {

@@ -276,6 +276,7 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
   __shared__ Real An[Q][TY + 2][AC];       // Zn(p+1/2): normal z-lerp between p and p+1
   __shared__ Real Cx[ADJ ? 1 : 10][TY][TX + 1];  // crossing populations of -x faces
   __shared__ Real Dy[ADJ ? 1 : 10][TY + 1][TX];  // crossing populations of -y faces
+  __shared__ Real Lam[ADJ ? Q : 1][TY + 2][AC];  // adjoint-face pass: lam'(p) of the tile and halo
   __shared__ Real WX[6][TX + 2];           // weights per tile column
   __shared__ Real WY[6][TY + 2];           // weights per tile row
 
@@ -400,14 +401,27 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
   };
   // adjoint-face mode: lam' of population ip at an interior cell (0 for a
   // ghost, cell < 0) and the store of a face's cotangents
-  auto lam_at = [&](int ip, long long cell) -> Real {
-    return cell < 0 ? Real(0) : (Real)adj.lam[(long long)ip * P.N + cell];
-  };
-  auto store_face = [&](int a, long long slot, const Real (&lam)[Q]) {
-    Real* out = static_cast<Real*>(adj.faces) + (long long)a * Q * P.eN + slot;
+  // (both arrays are < 2 GiB, checked at launch: 32-bit buffer offsets)
+  const RawBuf lamb(adj.lam, (unsigned)(Q * P.N * 8));
+  const unsigned lam_pop = (unsigned)(P.N * 8);                 // bytes per population
+  const unsigned face_pop = (unsigned)(P.eN * sizeof(Real));    // bytes per population
+  // lam' of the thread's own column at plane k (0 for a wall ghost column)
+  const bool own_cell = active && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+  auto load_lam = [&](int k, Real (&l)[Q]) {
+    const unsigned vb = own_cell ? (unsigned)(((long long)k * ny + gj) * nx + gi) * 8u : kOutOfRange;
     static_for<Q>([&](auto I) {
       constexpr int ip = decltype(I)::value;
-      out[(long long)ip * P.eN] = lam[ip];
+      if constexpr (ip != 0) l[ip] = (Real)lamb.ld_f64(vb, ip * lam_pop);
+    });
+  };
+  Real lamc[Q], lamn[Q];
+  if constexpr (ADJ) load_lam(z0, lamc);
+  auto store_face = [&](int a, long long slot, const Real (&lam)[Q]) {
+    const RawBuf fb(static_cast<Real*>(adj.faces) + (long long)a * Q * P.eN, Q * face_pop);
+    const unsigned vb = (unsigned)slot * (unsigned)sizeof(Real);
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      fb.st(lam[ip], vb, ip * face_pop);
     });
   };
   const long long e0 = P.e[0], e1 = P.e[1];
@@ -433,6 +447,7 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
 
   for (int p = z0; p < zend; p++) {
     const int kn = (p + 1) % nz;
+    if constexpr (ADJ) load_lam(kn, lamn);
     if (active) fbar_column<Real, Store>(P, bin, Nb, gi, gj, kn, Famb, fbn);
     // ---- phase 1: Z(p) (tangential z-lerp of f-bar(p)) and Zn(p+1/2) ----
     if (active) {
@@ -450,6 +465,14 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
       });
     }
     store_zn(fbc, fbn);
+    if constexpr (ADJ) {
+      if (active) {
+        static_for<Q>([&](auto I) {
+          constexpr int ip = decltype(I)::value;
+          if constexpr (ip != 0) Lam[ip][r][c + 1] = lamc[ip];
+        });
+      }
+    }
     __syncthreads();
     // ---- phase 2: the faces of plane p (-x, -y from Z; +z from Zn) ----
     // the forward needs the tile's right and top boundary faces too; the
@@ -479,15 +502,12 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
       const bool lw = !P.periodic[0] && gi == 0, hw = !P.periodic[0] && gi == nx;
       if (!P.periodic[0]) fc.bounce(lw, hw);
       if constexpr (ADJ) {
-        const long long rowc = ((long long)p * ny + gj) * nx;
-        const long long cl = gi > 0 ? rowc + gi - 1 : (P.periodic[0] ? rowc + nx - 1 : -1);
-        const long long cr = gi < nx ? rowc + gi : -1;
         const Real uL = WX[5][c], uR = WX[5][c + 1];
         Real mu[10], lam[Q];
         static_for<10>([&](auto K) {
           constexpr int k = decltype(K)::value;
           constexpr int ip = XD[k];
-          mu[k] = Real(CX[ip]) * (uR * lam_at(ip, cr) - uL * lam_at(ip, cl));
+          mu[k] = Real(CX[ip]) * (uR * Lam[ip][r][c + 1] - uL * Lam[ip][r][c]);
         });
         fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, lw, hw, lam);
         const int slot = (gi == 0 && P.periodic[0]) ? nx : gi;
@@ -523,15 +543,12 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
       const bool lw = !P.periodic[1] && gj == 0, hw = !P.periodic[1] && gj == ny;
       if (!P.periodic[1]) fc.bounce(lw, hw);
       if constexpr (ADJ) {
-        const long long pl = (long long)p * ny;
-        const long long cl = gj > 0 ? (pl + gj - 1) * nx + gi : (P.periodic[1] ? (pl + ny - 1) * nx + gi : -1);
-        const long long cr = gj < ny ? (pl + gj) * nx + gi : -1;
         const Real uL = WY[5][r - 1], uR = WY[5][r];
         Real mu[10], lam[Q];
         static_for<10>([&](auto K) {
           constexpr int k = decltype(K)::value;
           constexpr int ip = YD[k];
-          mu[k] = Real(CY[ip]) * (uR * lam_at(ip, cr) - uL * lam_at(ip, cl));
+          mu[k] = Real(CY[ip]) * (uR * Lam[ip][r][c + 1] - uL * Lam[ip][r - 1][c + 1]);
         });
         fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, lw, hw, lam);
         const int slot = (gj == 0 && P.periodic[1]) ? ny : gj;
@@ -551,13 +568,11 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
         // +z face of the cell: between planes p and p+1 (z is periodic)
         Face<Real, 2> fc;
         zface_rec(fc);
-        const long long cl = ((long long)p * ny + gj) * nx + gi;
-        const long long cr = ((long long)((p + 1) % nz) * ny + gj) * nx + gi;
         Real mu[10], lam[Q];
         static_for<10>([&](auto K) {
           constexpr int k = decltype(K)::value;
           constexpr int ip = ZD[k];
-          mu[k] = Real(CZ[ip]) * updz * (lam_at(ip, cr) - lam_at(ip, cl));
+          mu[k] = Real(CZ[ip]) * updz * (lamn[ip] - lamc[ip]);
         });
         fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, false, false, lam);
         store_face(2, ((long long)(p + 1) * e1 + (gj + 1)) * e0 + (gi + 1), lam);
@@ -612,6 +627,7 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
       static_for<Q>([&](auto I) {
         constexpr int ip = decltype(I)::value;
         fbc[ip] = fbn[ip];
+        if constexpr (ADJ && ip != 0) lamc[ip] = lamn[ip];
       });
     }
     // no barrier: the next phase 1 writes A/An, which phase 3 does not read,
@@ -737,6 +753,9 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const double* lam, 
   int tx = 32, ty = 4;
   *handled = false;
   if (c.kernel == 0 || !fused_supported(c, &tx, &ty)) return hipSuccess;
+  // lam' and each face array are addressed with 32-bit buffer offsets
+  const double rb = c.precision == 0 ? 8.0 : 4.0;
+  if ((double)Q * c.kp.N * 8.0 >= 2147483648.0 || (double)Q * c.kp.eN * rb >= 2147483648.0) return hipSuccess;
   *handled = true;
   switch (c.precision) {
     case 0:
