@@ -678,7 +678,7 @@ template <class Real, class Store, int TX, int TY>
 hipError_t run_fused(Ctx& cx, const void* fin, void* fout) {
   const KParams& P = cx.kp;
   using T = Tile<TX, TY>;
-  const int TZ = cx.fused_tz > 0 ? cx.fused_tz : 32;
+  const int TZ = cx.fused_tz > 0 ? cx.fused_tz : 64;
   dim3 grid(P.n[0] / TX, P.n[1] / TY, (P.n[2] + TZ - 1) / TZ);
   const long long nblocks = (long long)grid.x * grid.y * grid.z;
   double* partial = nullptr;
@@ -705,7 +705,7 @@ template <class Real, class Store, int TX, int TY>
 hipError_t run_adj_faces(Ctx& cx, const void* f, const double* lam, void* faces) {
   const KParams& P = cx.kp;
   using T = Tile<TX, TY>;
-  const int TZ = cx.fused_tz > 0 ? cx.fused_tz : 32;
+  const int TZ = cx.fused_tz > 0 ? cx.fused_tz : 64;
   dim3 grid(P.n[0] / TX, P.n[1] / TY, (P.n[2] + TZ - 1) / TZ);
   hipLaunchKernelGGL((k_fused<Real, Store, TX, TY, true>), grid, dim3(T::NT), 0, cx.stream, P,
                      static_cast<const Store*>(f), static_cast<Store*>(nullptr), TZ, nullptr,
@@ -724,9 +724,10 @@ bool fused_supported(const Ctx& c, int* tx, int* ty) {
   // lane and the kernel then spills, which costs more than the wider tile's
   // smaller halo saves
   *tx = 32;
-  // FP64 state needs twice the registers: a 4-row tile (4 waves, one per
-  // SIMD) runs without spills, an 8-row tile (6 waves) spills
-  *ty = c.precision == 0 ? 4 : 8;
+  // 4-row tiles (4 waves): FP64 runs one wave per SIMD without spills, the
+  // FP32 kernels (about 200 registers per lane) fit two such blocks per CU,
+  // which hides more latency than the 8-row tile's smaller halo saves
+  *ty = 4;
   if (s.n[0] % *tx != 0 || s.n[1] % *ty != 0) return false;
   // a wall axis needs two interior cells per side of the extrapolation
   if (!s.periodic[0] && s.n[0] < 3) return false;
@@ -743,9 +744,9 @@ hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled)
     case 0:
       return run_fused<double, double, 32, 4>(c, fin, fout);
     case 1:
-      return run_fused<float, float, 32, 8>(c, fin, fout);
+      return run_fused<float, float, 32, 4>(c, fin, fout);
     default:
-      return run_fused<float, __half, 32, 8>(c, fin, fout);
+      return run_fused<float, __half, 32, 4>(c, fin, fout);
   }
 }
 
@@ -761,9 +762,9 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const double* lam, 
     case 0:
       return run_adj_faces<double, double, 32, 4>(c, f, lam, faces);
     case 1:
-      return run_adj_faces<float, float, 32, 8>(c, f, lam, faces);
+      return run_adj_faces<float, float, 32, 4>(c, f, lam, faces);
     default:
-      return run_adj_faces<float, __half, 32, 8>(c, f, lam, faces);
+      return run_adj_faces<float, __half, 32, 4>(c, f, lam, faces);
   }
 }
 

@@ -51,8 +51,8 @@ CASES = {
     "walls_z": dict(nx=8, ny=6, nz=10, periods_z=0, ambient_fy=2e-4, xwidth=0.8, ywidth=0.6, zwidth=1.0),
     # cases the fused adjoint-face pass covers (nx % 32 == 0, z periodic)
     "walls_x": dict(nx=32, ny=8, nz=4, periods_x=0, ambient_fz=2e-4, xwidth=3.2, ywidth=0.8, zwidth=0.4),
-    "tiles": dict(nx=64, ny=16, nz=40, periods_y=0, mesh_y="tanh:0.85", dt_rule="min", ambient_fx=2e-4,
-                  xwidth=6.4, ywidth=2.0, zwidth=4.0),
+    "tiles": dict(nx=64, ny=16, nz=80, periods_y=0, mesh_y="tanh:0.85", dt_rule="min", ambient_fx=2e-4,
+                  xwidth=6.4, ywidth=2.0, zwidth=8.0),
 }
 FUSED_CASES = ("periodic", "walls_y", "walls_x", "tiles")
 

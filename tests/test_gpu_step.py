@@ -44,7 +44,7 @@ def _relerr(a, b):
 
 
 # cases both step kernels cover (the fused kernel needs nx % 32 == 0,
-# ny % 8 == 0 and a periodic uniform z axis)
+# ny % 4 == 0 and a periodic uniform z axis)
 CASES = {
     "periodic": dict(nx=32, ny=8, nz=4, xwidth=3.2, ywidth=0.8, zwidth=0.4),
     "periodic_wide": dict(nx=64, ny=8, nz=3, xwidth=6.4, ywidth=0.8, zwidth=0.3),
@@ -59,9 +59,9 @@ CASES = {
     "force_all": dict(nx=32, ny=8, nz=4, ambient_fx=1e-4, ambient_fy=-2e-4, ambient_fz=3e-4,
                       xwidth=3.2, ywidth=0.8, zwidth=0.4),
     # a multiple of 8 tiles (the XCD-ordered tile mapping) with a partial
-    # last z chunk (40 = 32 + 8)
-    "tiles_xcd": dict(nx=64, ny=16, nz=40, periods_y=0, mesh_y="tanh:0.85", dt_rule="min",
-                      ambient_fx=2e-4, xwidth=6.4, ywidth=2.0, zwidth=4.0),
+    # last z chunk (80 = 64 + 16)
+    "tiles_xcd": dict(nx=64, ny=16, nz=80, periods_y=0, mesh_y="tanh:0.85", dt_rule="min",
+                      ambient_fx=2e-4, xwidth=6.4, ywidth=2.0, zwidth=8.0),
 }
 # configurations only the two-pass kernels cover
 TWOPASS_CASES = {
