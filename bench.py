@@ -193,6 +193,68 @@ def cpu_baseline(configs, threads):
                        f"({cells} cells, same stretched mesh and dt), {dt:.1f} s")
 
 
+def study_tgv(dugks_amd, configs, torch, np, steps, every=10):
+    """3D Taylor-Green vortex, 128^3 periodic, u0 = 0.05, Re = 1600: kinetic
+    energy E(t) (the device loss functional, 0.5 <rho |u|^2>) every `every`
+    steps and the dissipation rate eps = -dE/dt (central differences), FP32
+    against FP64."""
+    series = {}
+    rates = {}
+    for m in ("fp64", "fp32"):
+        prec = {"fp64": dugks_amd.PREC_FP64, "fp32": dugks_amd.PREC_FP32}[m]
+        eng = dugks_amd.Engine(configs.taylor_green(n=128, precision=m, nsteps=steps), precision=prec, device=0,
+                               stream=torch.cuda.current_stream().cuda_stream)
+        E = [eng.loss(dugks_amd.LOSS_KINETIC_ENERGY)]
+        t0 = time.perf_counter()
+        for k in range(steps // every):
+            eng.step(every, istep0=1 + k * every)
+            E.append(eng.loss(dugks_amd.LOSS_KINETIC_ENERGY))
+        torch.cuda.synchronize()
+        rates[m] = eng.ncell * steps / (time.perf_counter() - t0) / 1e6
+        dt = eng.params["dt"]
+        eng.close()
+        series[m] = np.array(E), dt
+        log(f"tgv {m}: E {E[0]:.6e} -> {E[-1]:.6e}")
+    E64, dt = series["fp64"]
+    E32, _ = series["fp32"]
+    eps64 = -(E64[2:] - E64[:-2]) / (2 * every * dt)
+    eps32 = -(E32[2:] - E32[:-2]) / (2 * every * dt)
+    return {"workload": "Taylor-Green vortex 128^3, u0=0.05, Re=1600", "steps": steps,
+            "sample_every": every,
+            "E_max_rel_err": float(np.abs(E32 - E64).max() / np.abs(E64).max()),
+            "eps_max_rel_err": float(np.abs(eps32 - eps64).max() / np.abs(eps64).max()),
+            "E_final": {"fp64": float(E64[-1]), "fp32": float(E32[-1])},
+            "eps_final": {"fp64": float(eps64[-1]), "fp32": float(eps32[-1])},
+            "mlups_incl_sampling": {k: round(v, 1) for k, v in rates.items()}}
+
+
+def study_channel(dugks_amd, configs, torch, np, steps):
+    """Re_tau = 180 channel 128^3 (tanh-stretched y, constant flow rate,
+    synthetic log-law + Fourier state): mean-profile, rms and Reynolds-stress
+    errors of FP32 and FP16-store against FP64 after `steps` steps."""
+    stats, rates = {}, {}
+    for m in ("fp64", "fp32", "fp16"):
+        prec = {"fp64": dugks_amd.PREC_FP64, "fp32": dugks_amd.PREC_FP32, "fp16": dugks_amd.PREC_FP16S}[m]
+        eng = dugks_amd.Engine(configs.turbulent_channel(precision=m, nsteps=steps), precision=prec, device=0,
+                               stream=torch.cuda.current_stream().cuda_stream)
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:
+            k = min(1000, steps - done)
+            eng.step(k, istep0=1 + done)
+            done += k
+            torch.cuda.synchronize()
+            log(f"channel {m}: {done} steps")
+        rates[m] = eng.ncell * steps / (time.perf_counter() - t0) / 1e6
+        stats[m] = field_stats(eng, np)
+        eng.close()
+    out = {"workload": "Re_tau=180 channel 128^3, tanh y (alpha 0.85), constant flow rate", "steps": steps,
+           "against": "fp64", "mlups": {k: round(v, 1) for k, v in rates.items()}}
+    for m in ("fp32", "fp16"):
+        out[m] = {k: float(f"{v:.3e}") for k, v in field_errors(stats["fp64"], stats[m], np).items()}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,6 +266,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--adj-steps", type=int, default=100)
+    ap.add_argument("--study", choices=["tgv", "channel"],
+                    help="instead of the benchmark: the FP32 / FP16-store error study on the "
+                         "Taylor-Green vortex or the 128^3 channel (prints one JSON line)")
+    ap.add_argument("--study-steps", type=int, default=0, help="steps of the study (default 1000 / 10000)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB read + 1 GiB write elementwise kernel 5 times "
                          "(the reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -224,6 +290,14 @@ def main():
         return
     import dugks_amd
     from dugks_amd import configs
+    if args.study:
+        import numpy as np
+        if args.study == "tgv":
+            res = study_tgv(dugks_amd, configs, torch, np, args.study_steps or 1000)
+        else:
+            res = study_channel(dugks_amd, configs, torch, np, args.study_steps or 10000)
+        print(json.dumps({"study": args.study, **res}), flush=True)
+        return
 
     import numpy as np
     modes = [m for m in args.modes.split(",") if m]

@@ -674,11 +674,22 @@ __global__ void __launch_bounds__(256) k_flow_finish(const double* partial, int 
   }
 }
 
+// z-chunk length: 64 planes (fewer re-computed chunk boundaries) unless that
+// leaves fewer than 4 tiles per CU to fill the device (256 CUs, two or three
+// resident blocks each); then halve it, down to 8.
+int z_chunk(const Ctx& cx, int TX, int TY) {
+  if (cx.fused_tz > 0) return cx.fused_tz;
+  const long long cols = (long long)(cx.kp.n[0] / TX) * (cx.kp.n[1] / TY);
+  int tz = 64;
+  while (tz > 8 && cols * ((cx.kp.n[2] + tz - 1) / tz) < 4 * 256) tz /= 2;
+  return tz;
+}
+
 template <class Real, class Store, int TX, int TY>
 hipError_t run_fused(Ctx& cx, const void* fin, void* fout) {
   const KParams& P = cx.kp;
   using T = Tile<TX, TY>;
-  const int TZ = cx.fused_tz > 0 ? cx.fused_tz : 64;
+  const int TZ = z_chunk(cx, TX, TY);
   dim3 grid(P.n[0] / TX, P.n[1] / TY, (P.n[2] + TZ - 1) / TZ);
   const long long nblocks = (long long)grid.x * grid.y * grid.z;
   double* partial = nullptr;
@@ -705,7 +716,7 @@ template <class Real, class Store, int TX, int TY>
 hipError_t run_adj_faces(Ctx& cx, const void* f, const double* lam, void* faces) {
   const KParams& P = cx.kp;
   using T = Tile<TX, TY>;
-  const int TZ = cx.fused_tz > 0 ? cx.fused_tz : 64;
+  const int TZ = z_chunk(cx, TX, TY);
   dim3 grid(P.n[0] / TX, P.n[1] / TY, (P.n[2] + TZ - 1) / TZ);
   hipLaunchKernelGGL((k_fused<Real, Store, TX, TY, true>), grid, dim3(T::NT), 0, cx.stream, P,
                      static_cast<const Store*>(f), static_cast<Store*>(nullptr), TZ, nullptr,
