@@ -791,3 +791,35 @@ void oracle_reconstruct(const oracle_ctx* o, const double* fbar, double* faces) 
 }
 
 }  // extern "C"
+
+/* Streak forcing of the reference, calc_perturb_force (src/dugks.cu:1086-1126),
+ * on one rank (zoffset 0): the total centre force of every interior cell at
+ * time step `step`; cells outside the forced band 3 < i < nx/4 + 2 (1-based i)
+ * carry the ambient force.  d[] are the uniform cell widths (n->dx, dy, dz). */
+extern "C" void oracle_perturb_force(int nx, int ny, int nz, const double width[3], const double d[3],
+                                     double dt, double visc, int step, const double famb[3], double* fx,
+                                     double* fy, double* fz) {
+  const double two_pi = 2.0 * 3.14159265358979323846;
+  const double beta = 3, Tpd = 2000, gamma = 2, Re_tau = 180;
+  const double ustar = 2 * Re_tau * visc / width[0];
+  const double FF = 2 * std::pow(ustar, 2) / width[0];
+  const double A0 = 5 * 40.0 / nx * beta * std::sin(two_pi * step * dt / Tpd);
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        const size_t c = ((size_t)(k - 1) * ny + (j - 1)) * nx + (i - 1);
+        if (3 < i && i < nx / 4 + 2) {
+          const double xx = two_pi * (i - 3) * d[0] / (width[0] / 4.);
+          const double yy = two_pi * (j - 0.5) * d[1] / width[1];
+          const double zz = two_pi * ((k - 0.5) * d[2]) / width[2];
+          fx[c] = 0.5 * FF * A0 * width[0] / 4 * (1 - std::cos(xx)) * std::cos(beta * yy) * std::cos(gamma * zz);
+          fy[c] = 0.5 * FF * A0 * width[1] / gamma * std::sin(xx) * std::cos(beta * yy) * std::sin(gamma * zz);
+          fz[c] = FF * (1 - A0 * width[2] / beta * std::sin(xx) * std::sin(beta * yy) * std::cos(gamma * zz));
+        } else {
+          fx[c] = famb[0];
+          fy[c] = famb[1];
+          fz[c] = famb[2];
+        }
+      }
+}
+

@@ -65,6 +65,12 @@ void oracle_reconstruct(const oracle_ctx*, const double* fbar, double* faces);
 void oracle_adjoint_step(const oracle_ctx*, const double* lam_next, double* lam_prev,
                          double* grad_force);
 
+/* Streak forcing of the reference (calc_perturb_force, src/dugks.cu:1086-1126):
+ * total centre force per interior cell at a time step, [k][j][i] arrays;
+ * ambient outside the forced band 3 < i < nx/4 + 2 (1-based). */
+void oracle_perturb_force(int nx, int ny, int nz, const double width[3], const double d[3], double dt,
+                          double visc, int step, const double famb[3], double* fx, double* fy, double* fz);
+
 #ifdef __cplusplus
 }
 #endif

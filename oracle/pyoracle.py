@@ -39,6 +39,8 @@ def _load():
         lib.oracle_coefficients.argtypes = [ctypes.c_void_p, _dp]
         lib.oracle_reconstruct.argtypes = [ctypes.c_void_p, _dp, _dp]
         lib.oracle_adjoint_step.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        lib.oracle_perturb_force.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, ctypes.c_double, ctypes.c_double,
+                                                                   ctypes.c_int, _dp, _dp, _dp, _dp]
         _lib = lib
     return _lib
 
@@ -128,3 +130,16 @@ class Oracle:
             grad_force = np.zeros((nz, ny, nx, 3))
         self._lib.oracle_adjoint_step(self._h, _ptr(lam_next), _ptr(lam_prev), _ptr(grad_force))
         return lam_prev, grad_force
+
+
+def perturb_force(shape, width, spacing, dt, visc, step, famb=(0.0, 0.0, 0.0)):
+    """Total centre force (fx, fy, fz), each [nz][ny][nx], of the reference's
+    streak forcing at time step `step` (calc_perturb_force,
+    src/dugks.cu:1086-1126)."""
+    lib = _load()
+    nx, ny, nz = shape
+    out = [np.zeros((nz, ny, nx)) for _ in range(3)]
+    lib.oracle_perturb_force(nx, ny, nz, _ptr(_c64(width)), _ptr(_c64(spacing)), float(dt), float(visc),
+                             int(step), _ptr(_c64(famb)), *[_ptr(a) for a in out])
+    return out
+

@@ -266,3 +266,26 @@ def test_outputs_written(tmp_path):
     rho, ux, uy, uz = eng.macro()
     first = [float(v) for v in rows[1].split()]
     assert abs(first[2] - uz[:, :, 0].mean()) < 1e-7 * abs(uz).max()
+
+
+@pytest.mark.parametrize("kernel", KERNELS, ids=["twopass", "fused"])
+def test_perturb_forcing_matches_oracle(rng, kernel):
+    # the reference's streak forcing (perturb = 1) recomputed every step on
+    # the device against the oracle's field applied as the centre force
+    from pyoracle import perturb_force
+    text = configs.base(nx=32, ny=8, nz=6, periods_y=0, visc=0.02, ambient_fx=2e-4, perturb=1,
+                        xwidth=3.2, ywidth=0.8, zwidth=0.6)
+    eng = _engine(text, kernel=kernel)
+    ora = _oracle_for(eng, text)
+    f0 = random_state(eng.shape, rng)
+    eng.set_f(f0)
+    ora.set_f(f0)
+    p = eng.params
+    for step in (37, 38):
+        force = perturb_force(eng.shape, (3.2, 0.8, 0.6), (p["dx"], p["dy"], p["dz"]), p["dt"], p["visc"],
+                              step, (2e-4, 0.0, 0.0))
+        assert np.abs(force[1]).max() > 0  # the forcing is on at these steps
+        ora.set_cell_force(*force)
+        eng.step(1, istep0=step)
+        ora.step(1)
+    assert _relerr(eng.get_f(), ora.get_f()) <= 1e-13

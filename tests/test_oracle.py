@@ -321,3 +321,30 @@ def test_oracle_adjoint_matches_central_differences(rng, variant):
         an = float((gF.transpose(3, 0, 1, 2) * w).sum())
         assert abs(an) > 1e-4
         assert abs(fd - an) <= 1e-6 * abs(an) + 1e-8
+
+
+def test_perturb_force_structure():
+    # the streak forcing (calc_perturb_force): ambient outside the band
+    # 3 < i < nx/4 + 2, the pure driving force FF = 2 u*^2 / Lx at step 0
+    # (amplitude A0 ~ sin(2 pi step dt / 2000) vanishes), and a perturbation
+    # that scales with that sine
+    from pyoracle import perturb_force
+    shape, width, d = (32, 8, 6), (3.2, 0.8, 0.6), (0.1, 0.1, 0.1)
+    dt, visc, famb = 0.5, 0.02, (1e-4, 2e-4, -3e-4)
+    band = np.zeros(32, bool)
+    band[3:9] = True  # 1-based i = 4..9
+    fx0, fy0, fz0 = perturb_force(shape, width, d, dt, visc, 0, famb)
+    ustar = 2 * 180 * visc / width[0]
+    FF = 2 * ustar ** 2 / width[0]
+    for f, a in zip((fx0, fy0, fz0), famb):
+        assert np.all(f[:, :, ~band] == a)
+    assert np.all(fx0[:, :, band] == 0) and np.all(fy0[:, :, band] == 0)
+    assert np.allclose(fz0[:, :, band], FF, rtol=1e-15)
+    s1, s2 = 100, 333
+    r = math.sin(2 * math.pi * s2 * dt / 2000) / math.sin(2 * math.pi * s1 * dt / 2000)
+    f1 = perturb_force(shape, width, d, dt, visc, s1, famb)
+    f2 = perturb_force(shape, width, d, dt, visc, s2, famb)
+    for a in range(2):
+        assert np.abs(f1[a][:, :, band]).max() > 0
+        assert np.allclose(f2[a][:, :, band], r * f1[a][:, :, band], rtol=1e-12, atol=1e-20)
+    assert np.allclose(f2[2][:, :, band] - FF, r * (f1[2][:, :, band] - FF), rtol=1e-12, atol=1e-12 * FF)
