@@ -260,9 +260,20 @@ __global__ void __launch_bounds__(256) k_adj_cells(KParams P, const Store* __res
                                                    const double* __restrict__ lam, const Real* __restrict__ faces,
                                                    double* __restrict__ lam_prev, double* __restrict__ grad,
                                                    int wall_axis) {
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= P.N) return;
-  const int t[3] = {(int)(c % P.n[0]) + 1, (int)((c / P.n[0]) % P.n[1]) + 1, (int)(c / ((long long)P.n[0] * P.n[1])) + 1};
+  // 32 x 8 cell tiles (a wave covers 32 x 2 cells, so y-neighbour faces are
+  // shared inside a wave), dealt to the XCDs in contiguous runs of tiles
+  // (x fastest) so that neighbouring tiles share an L2: each face value is
+  // gathered by cells of up to four tiles
+  const int gx = (P.n[0] + 31) / 32, gy = (P.n[1] + 7) / 8;
+  const int ntiles = gx * gy * P.n[2];
+  const int b = blockIdx.x;
+  const int tt = (ntiles % 8 == 0) ? (b % 8) * (ntiles / 8) + b / 8 : b;
+  const int ci = (tt % gx) * 32 + (int)threadIdx.x % 32;
+  const int cj = ((tt / gx) % gy) * 8 + (int)threadIdx.x / 32;
+  const int ck = tt / (gx * gy);
+  if (ci >= P.n[0] || cj >= P.n[1]) return;
+  const long long c = ((long long)ck * P.n[1] + cj) * P.n[0] + ci;
+  const int t[3] = {ci + 1, cj + 1, ck + 1};
   Real lfb[Q];
   static_for<Q>([&](auto I) {
     constexpr int ip = decltype(I)::value;
@@ -346,7 +357,7 @@ hipError_t run_adjoint(Ctx& cx, const void* f, const double* lam_next, double* l
     hipLaunchKernelGGL((k_face_ghosts<Real>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, cx.stream, P, faces,
                        b);
   }
-  const unsigned nbc = (unsigned)((P.N + 255) / 256);
+  const unsigned nbc = (unsigned)((P.n[0] + 31) / 32 * ((P.n[1] + 7) / 8) * P.n[2]);
   hipLaunchKernelGGL((k_adj_cells<Real, Store>), dim3(nbc), dim3(256), 0, cx.stream, P,
                      static_cast<const Store*>(f), lam_next, static_cast<const Real*>(cx.adj_faces), lam_prev, grad,
                      wall);

@@ -42,11 +42,11 @@ constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
 
 template <int TX, int TY>
 struct Tile {
-  static constexpr int NROW = (TY + 2) * TX;
-  static constexpr int NT = NROW + 64;
+  static constexpr int NROW = (TY + 2) * TX;    // the tile rows and the two y-halo rows
+  static constexpr int NHALO = 2 * (TY + 2);     // the two x-halo columns
+  static constexpr int NT = (NROW + NHALO + 63) / 64 * 64;
   static constexpr int AC = TX + 2;
-  static_assert(2 * (TY + 2) <= 64, "x-halo columns must fit one wave");
-  static_assert(TX == 32 || TX == 64, "rows are half or whole waves");
+  static_assert(TX % 16 == 0 && TX <= 64, "rows of 16, 32, 48 or 64 lanes");
 };
 
 template <class Real>
@@ -267,7 +267,8 @@ struct AdjFaceIO {
 // cotangents of its 19 reconstructed values, driven by lam' (adj.lam), are
 // written to adj.faces; nothing else is written.
 template <class Real, class Store, int TX, int TY, bool ADJ = false>
-__global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const Store* __restrict__ fin,
+__global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64) k_fused(
+    KParams P, const Store* __restrict__ fin,
                                                           Store* __restrict__ fout, int TZ,
                                                           double* __restrict__ partial, AdjFaceIO adj) {
   using T = Tile<TX, TY>;
@@ -300,19 +301,25 @@ __global__ void __launch_bounds__((TY + 2) * TX + 64) k_fused(KParams P, const S
   const StateBuf<Store> bin(fin, Q * Nb), bout(fout, Q * Nb);
 
   // column of this thread in tile coordinates: r = 0..TY+1 (y = j0-1+r),
-  // c = -1..TX (x = i0+c)
-  int r, c;
+  // c = -1..TX (x = i0+c).  Lanes: the TY interior rows first (whole waves
+  // when TY*TX is a multiple of 64, so the interior-only phases run on as few
+  // waves as possible), then the two y-halo rows, then the two x-halo columns
+  // packed behind them; lanes past those idle.
+  int r = 0, c = 0;
   bool active = true;
-  if (t < T::NROW) {
-    const int slot = t / TX;
+  if (t < TY * TX) {
+    r = t / TX + 1;
     c = t % TX;
-    r = slot == 0 ? 0 : (slot == 1 ? TY + 1 : slot - 1);
-  } else {
+  } else if (t < T::NROW) {
+    const int u = t - TY * TX;
+    r = u < TX ? 0 : TY + 1;
+    c = u % TX;
+  } else if (t < T::NROW + T::NHALO) {
     const int u = t - T::NROW;
-    active = u < 2 * (TY + 2);
-    const int side = u / (TY + 2);
     r = u % (TY + 2);
-    c = side ? TX : -1;
+    c = u < TY + 2 ? -1 : TX;
+  } else {
+    active = false;
   }
   const bool interior = t < T::NROW && r >= 1 && r <= TY;
   int gi = i0 + c, gj = j0 - 1 + r;
@@ -731,14 +738,18 @@ bool fused_supported(const Ctx& c, int* tx, int* ty) {
   if (s.recon != 0 || !s.periodic[2] || !s.uniform[2]) return false;
   // buffer offsets are 32-bit: one state array must span < 4 GiB
   if ((double)s.ncell() * Q * c.store_bytes >= 4294967296.0) return false;
-  // 32-wide tiles: the 64-wide tile's 11 waves cap registers at 168 per
-  // lane and the kernel then spills, which costs more than the wider tile's
-  // smaller halo saves
-  *tx = 32;
-  // 4-row tiles (4 waves): FP64 runs one wave per SIMD without spills, the
-  // FP32 kernels (about 200 registers per lane) fit two such blocks per CU,
-  // which hides more latency than the 8-row tile's smaller halo saves
-  *ty = 4;
+  // Tiles of 4 waves: 16 x 12 cells when the grid divides (1.31 f-bar
+  // evaluations per cell, the x-halo lanes packed behind the rows), else
+  // 32 x 4 (2.0 per cell).  Four waves per block keep FP64 at one wave per
+  // SIMD without spills and fit two FP32 blocks (about 200 registers per
+  // lane) on a CU.
+  if (s.n[0] % 16 == 0 && s.n[1] % 12 == 0) {
+    *tx = 16;
+    *ty = 12;
+  } else {
+    *tx = 32;
+    *ty = 4;
+  }
   if (s.n[0] % *tx != 0 || s.n[1] % *ty != 0) return false;
   // a wall axis needs two interior cells per side of the extrapolation
   if (!s.periodic[0] && s.n[0] < 3) return false;
@@ -751,13 +762,14 @@ hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled)
   *handled = false;
   if (!fused_supported(c, &tx, &ty)) return hipSuccess;
   *handled = true;
+  const bool tall = tx == 16;
   switch (c.precision) {
     case 0:
-      return run_fused<double, double, 32, 4>(c, fin, fout);
+      return tall ? run_fused<double, double, 16, 12>(c, fin, fout) : run_fused<double, double, 32, 4>(c, fin, fout);
     case 1:
-      return run_fused<float, float, 32, 4>(c, fin, fout);
+      return tall ? run_fused<float, float, 16, 12>(c, fin, fout) : run_fused<float, float, 32, 4>(c, fin, fout);
     default:
-      return run_fused<float, __half, 32, 4>(c, fin, fout);
+      return tall ? run_fused<float, __half, 16, 12>(c, fin, fout) : run_fused<float, __half, 32, 4>(c, fin, fout);
   }
 }
 
@@ -769,13 +781,17 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const double* lam, 
   const double rb = c.precision == 0 ? 8.0 : 4.0;
   if ((double)Q * c.kp.N * 8.0 >= 2147483648.0 || (double)Q * c.kp.eN * rb >= 2147483648.0) return hipSuccess;
   *handled = true;
+  const bool tall = tx == 16;
   switch (c.precision) {
     case 0:
-      return run_adj_faces<double, double, 32, 4>(c, f, lam, faces);
+      return tall ? run_adj_faces<double, double, 16, 12>(c, f, lam, faces)
+                  : run_adj_faces<double, double, 32, 4>(c, f, lam, faces);
     case 1:
-      return run_adj_faces<float, float, 32, 4>(c, f, lam, faces);
+      return tall ? run_adj_faces<float, float, 16, 12>(c, f, lam, faces)
+                  : run_adj_faces<float, float, 32, 4>(c, f, lam, faces);
     default:
-      return run_adj_faces<float, __half, 32, 4>(c, f, lam, faces);
+      return tall ? run_adj_faces<float, __half, 16, 12>(c, f, lam, faces)
+                  : run_adj_faces<float, __half, 32, 4>(c, f, lam, faces);
   }
 }
 

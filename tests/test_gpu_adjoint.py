@@ -53,8 +53,12 @@ CASES = {
     "walls_x": dict(nx=32, ny=8, nz=4, periods_x=0, ambient_fz=2e-4, xwidth=3.2, ywidth=0.8, zwidth=0.4),
     "tiles": dict(nx=64, ny=16, nz=80, periods_y=0, mesh_y="tanh:0.85", dt_rule="min", ambient_fx=2e-4,
                   xwidth=6.4, ywidth=2.0, zwidth=8.0),
+    # 16 x 12 tiles
+    "tall": dict(nx=32, ny=24, nz=10, periods_y=0, mesh_y="tanh:0.85", dt_rule="min", ambient_fx=2e-4,
+                 xwidth=3.2, ywidth=2.0, zwidth=1.0),
+    "tall_walls_x": dict(nx=16, ny=12, nz=6, periods_x=0, ambient_fz=2e-4, xwidth=1.6, ywidth=1.2, zwidth=0.6),
 }
-FUSED_CASES = ("periodic", "walls_y", "walls_x", "tiles")
+FUSED_CASES = ("periodic", "walls_y", "walls_x", "tiles", "tall", "tall_walls_x")
 
 
 @pytest.mark.parametrize("case", list(CASES))

@@ -43,8 +43,8 @@ def _relerr(a, b):
     return float(np.abs(a - b).max() / np.abs(b).max())
 
 
-# cases both step kernels cover (the fused kernel needs nx % 32 == 0,
-# ny % 4 == 0 and a periodic uniform z axis)
+# cases both step kernels cover (the fused kernel needs nx % 32 == 0 and
+# ny % 4 == 0, or nx % 16 == 0 and ny % 12 == 0, and a periodic uniform z axis)
 CASES = {
     "periodic": dict(nx=32, ny=8, nz=4, xwidth=3.2, ywidth=0.8, zwidth=0.4),
     "periodic_wide": dict(nx=64, ny=8, nz=3, xwidth=6.4, ywidth=0.8, zwidth=0.3),
@@ -62,6 +62,14 @@ CASES = {
     # last z chunk (80 = 64 + 16)
     "tiles_xcd": dict(nx=64, ny=16, nz=80, periods_y=0, mesh_y="tanh:0.85", dt_rule="min",
                       ambient_fx=2e-4, xwidth=6.4, ywidth=2.0, zwidth=8.0),
+    # the 16 x 12 tile (ny % 12 == 0): walls and stretching in y, walls in x,
+    # several tiles in x and y with XCD ordering
+    "tall_walls_y": dict(nx=32, ny=24, nz=6, periods_y=0, mesh_y="tanh:0.85", dt_rule="min",
+                         ambient_fx=2e-4, xwidth=3.2, ywidth=2.0, zwidth=0.6),
+    "tall_walls_x": dict(nx=16, ny=12, nz=5, periods_x=0, ambient_fz=2e-4, xwidth=1.6, ywidth=1.2,
+                         zwidth=0.5),
+    "tall_tiles": dict(nx=64, ny=24, nz=40, periods_y=0, ambient_fx=1e-4, ambient_fz=-1e-4, xwidth=6.4,
+                       ywidth=2.4, zwidth=4.0),
 }
 # configurations only the two-pass kernels cover
 TWOPASS_CASES = {
