@@ -452,10 +452,36 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
     __syncthreads();
   }
 
+  // f of the own column two planes ahead (non-ghost columns): the loads are
+  // issued one iteration before their f-bar needs them (FP16 storage gains
+  // nothing from it and loads at use)
+  constexpr bool kPrefetch = !std::is_same<Store, __half>::value;
+  const bool own = kPrefetch && active && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+  Real fpre[Q];
+  auto prefetch = [&](int k) {
+    const unsigned vb = (((unsigned)k * ny + gj) * nx + gi) * (unsigned)sizeof(Store);
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      fpre[ip] = ld_state<Real, Store>(bin, vb, ip * Nb, P, ip);
+    });
+  };
+  if (own) prefetch((z0 + 1) % nz);
   for (int p = z0; p < zend; p++) {
     const int kn = (p + 1) % nz;
     if constexpr (ADJ) load_lam(kn, lamn);
-    if (active) fbar_column<Real, Store>(P, bin, Nb, gi, gj, kn, Famb, fbn);
+    if (own) {
+      Real F[3] = {Famb[0], Famb[1], Famb[2]};
+      if (P.fcell) {
+        const unsigned cc = ((unsigned)kn * ny + gj) * nx + gi;
+        F[0] += (Real)P.fcell[cc];
+        F[1] += (Real)P.fcell[P.N + cc];
+        F[2] += (Real)P.fcell[2 * P.N + cc];
+      }
+      centre_fbar<Real>(fpre, F, hdt, (Real)P.bw0, (Real)P.bw1, (Real)P.bw2, fbn);
+      if (p + 1 < zend) prefetch((p + 2) % nz);
+    } else if (active) {
+      fbar_column<Real, Store>(P, bin, Nb, gi, gj, kn, Famb, fbn);
+    }
     // ---- phase 1: Z(p) (tangential z-lerp of f-bar(p)) and Zn(p+1/2) ----
     if (active) {
       static_for<Q>([&](auto I) {
@@ -482,6 +508,17 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
     }
     __syncthreads();
     // ---- phase 2: the faces of plane p (-x, -y from Z; +z from Zn) ----
+    // f~(p) for the update, loaded now so that the face work hides the wait
+    Real fo[Q];
+    if constexpr (!ADJ) {
+      if (interior) {
+        const unsigned vb0 = (((unsigned)p * ny + gj) * nx + gi) * (unsigned)sizeof(Store);
+        static_for<Q>([&](auto I) {
+          constexpr int ip = decltype(I)::value;
+          fo[ip] = ld_state<Real, Store>(bin, vb0, ip * Nb, P, ip);
+        });
+      }
+    }
     // the forward needs the tile's right and top boundary faces too; the
     // adjoint-face pass leaves them to the next tile except at a wall
     const bool xhalo = active && t >= T::NROW && c == TX && r >= 1 && r <= TY;
@@ -598,8 +635,7 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
         Real fnew[Q];
         static_for<Q>([&](auto I) {
           constexpr int ip = decltype(I)::value;
-          const Real fo = ld_state<Real, Store>(bin, vb, ip * Nb, P, ip);
-          Real v = fbc[ip] * Real(4.0 / 3.0) - fo * Real(1.0 / 3.0);
+          Real v = fbc[ip] * Real(4.0 / 3.0) - fo[ip] * Real(1.0 / 3.0);
           constexpr int kx = cross_index<0>(ip), ky = cross_index<1>(ip), kz = cross_index<2>(ip);
           if constexpr (kx >= 0) v -= Real(CX[ip]) * updx * (Cx[kx][r - 1][c + 1] - Cx[kx][r - 1][c]);
           if constexpr (ky >= 0) v -= Real(CY[ip]) * updy * (Dy[ky][r][c] - Dy[ky][r - 1][c]);
