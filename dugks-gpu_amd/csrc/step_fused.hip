@@ -273,11 +273,18 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
                                                           double* __restrict__ partial, AdjFaceIO adj) {
   using T = Tile<TX, TY>;
   constexpr int AC = T::AC;
-  __shared__ Real A[Q][TY + 2][AC];        // Z(p): tangential z-lerp of f-bar(p)
-  __shared__ Real An[Q][TY + 2][AC];       // Zn(p+1/2): normal z-lerp between p and p+1
-  __shared__ Real Cx[ADJ ? 1 : 10][TY][TX + 1];  // crossing populations of -x faces
-  __shared__ Real Dy[ADJ ? 1 : 10][TY + 1][TX];  // crossing populations of -y faces
-  __shared__ Real Lam[ADJ ? Q : 1][TY + 2][AC];  // adjoint-face pass: lam'(p) of the tile and halo
+  // LDS planes indexed (row, column).  16-wide rows put four rows in a wave:
+  // column-major planes (column stride TY + 2, even) then keep the lanes of
+  // a ds_read on distinct banks; 32-wide rows are conflict-free row-major.
+  constexpr bool CM = TX == 16;
+  auto ia = [](int rr, int cc) { return CM ? cc * (TY + 2) + rr : rr * AC + cc; };       // A, An, Lam
+  auto ic = [](int rr, int cc) { return CM ? cc * TY + rr : rr * (TX + 1) + cc; };       // Cx (rows 0..TY-1)
+  auto id = [](int rr, int cc) { return CM ? cc * (TY + 1) + rr : rr * TX + cc; };       // Dy (rows 0..TY)
+  __shared__ Real A[Q][(TY + 2) * AC];        // Z(p): tangential z-lerp of f-bar(p)
+  __shared__ Real An[Q][(TY + 2) * AC];       // Zn(p+1/2): normal z-lerp between p and p+1
+  __shared__ Real Cx[ADJ ? 1 : 10][TY * (TX + 1)];  // crossing populations of -x faces
+  __shared__ Real Dy[ADJ ? 1 : 10][(TY + 1) * TX];  // crossing populations of -y faces
+  __shared__ Real Lam[ADJ ? Q : 1][(TY + 2) * AC];  // adjoint-face pass: lam'(p) of the tile and halo
   __shared__ Real WX[6][TX + 2];           // weights per tile column
   __shared__ Real WY[6][TY + 2];           // weights per tile row
 
@@ -374,9 +381,9 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
     static_for<Q>([&](auto I) {
       constexpr int ip = decltype(I)::value;
       if constexpr (ip == 0)
-        An[0][r][c + 1] = Real(0.5) * (lo[0] + hi[0]);
+        An[0][ia(r, c + 1)] = Real(0.5) * (lo[0] + hi[0]);
       else
-        An[ip][r][c + 1] = lerp_(lo[ip], hi[ip], wzn[CZ[ip] + 1]);
+        An[ip][ia(r, c + 1)] = lerp_(lo[ip], hi[ip], wzn[CZ[ip] + 1]);
     });
   };
   // +z face of an interior column from Zn in A: bilinear over the upwind
@@ -385,18 +392,18 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
     static_for<Q>([&](auto I) {
       constexpr int ip = decltype(I)::value;
       constexpr int cx = CX[ip], cy = CY[ip];
-      Real v = An[ip][r][c + 1];
+      Real v = An[ip][ia(r, c + 1)];
       if constexpr (cx != 0) {
         const Real w = xt(cx < 0 ? 0 : 1);
-        v = lerp_(v, An[ip][r][c + 1 - cx], w);
+        v = lerp_(v, An[ip][ia(r, c + 1 - cx)], w);
         if constexpr (cy != 0) {
           const Real wy = yt(cy < 0 ? 0 : 1);
-          const Real v2 = lerp_(An[ip][r - cy][c + 1], An[ip][r - cy][c + 1 - cx], w);
+          const Real v2 = lerp_(An[ip][ia(r - cy, c + 1)], An[ip][ia(r - cy, c + 1 - cx)], w);
           v = lerp_(v, v2, wy);
         }
       } else if constexpr (cy != 0) {
         const Real wy = yt(cy < 0 ? 0 : 1);
-        v = lerp_(v, An[ip][r - cy][c + 1], wy);
+        v = lerp_(v, An[ip][ia(r - cy, c + 1)], wy);
       }
       fc.template put<ip>(v);
     });
@@ -488,12 +495,12 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
         constexpr int ip = decltype(I)::value;
         constexpr int cz = CZ[ip];
         if constexpr (cz == 0) {
-          A[ip][r][c + 1] = fbc[ip];
+          A[ip][ia(r, c + 1)] = fbc[ip];
         } else if constexpr (cz > 0) {
           constexpr int zk = ip == 5 ? 0 : (ip == 11 ? 1 : (ip == 12 ? 2 : (ip == 15 ? 3 : 4)));
-          A[ip][r][c + 1] = lerp_(fbc[ip], fbm[zk], wzt_p);
+          A[ip][ia(r, c + 1)] = lerp_(fbc[ip], fbm[zk], wzt_p);
         } else {
-          A[ip][r][c + 1] = lerp_(fbc[ip], fbn[ip], wzt_m);
+          A[ip][ia(r, c + 1)] = lerp_(fbc[ip], fbn[ip], wzt_m);
         }
       });
     }
@@ -502,7 +509,7 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
       if (active) {
         static_for<Q>([&](auto I) {
           constexpr int ip = decltype(I)::value;
-          if constexpr (ip != 0) Lam[ip][r][c + 1] = lamc[ip];
+          if constexpr (ip != 0) Lam[ip][ia(r, c + 1)] = lamc[ip];
         });
       }
     }
@@ -531,13 +538,13 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
         constexpr int ip = decltype(I)::value;
         constexpr int cx = CX[ip], cy = CY[ip];
         if constexpr (ip == 0) {
-          fc.template put<0>(Real(0.5) * (A[0][r][c] + A[0][r][c + 1]));
+          fc.template put<0>(Real(0.5) * (A[0][ia(r, c)] + A[0][ia(r, c + 1)]));
         } else {
           const Real w = xn(cx + 1);  // fraction toward the right cell
-          Real v = lerp_(A[ip][r][c], A[ip][r][c + 1], w);
+          Real v = lerp_(A[ip][ia(r, c)], A[ip][ia(r, c + 1)], w);
           if constexpr (cy != 0) {
             const Real wy = yt(cy < 0 ? 0 : 1);
-            const Real v2 = lerp_(A[ip][r - cy][c], A[ip][r - cy][c + 1], w);
+            const Real v2 = lerp_(A[ip][ia(r - cy, c)], A[ip][ia(r - cy, c + 1)], w);
             v = lerp_(v, v2, wy);
           }
           fc.template put<ip>(v);
@@ -551,7 +558,7 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
         static_for<10>([&](auto K) {
           constexpr int k = decltype(K)::value;
           constexpr int ip = XD[k];
-          mu[k] = Real(CX[ip]) * (uR * Lam[ip][r][c + 1] - uL * Lam[ip][r][c]);
+          mu[k] = Real(CX[ip]) * (uR * Lam[ip][ia(r, c + 1)] - uL * Lam[ip][ia(r, c)]);
         });
         fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, lw, hw, lam);
         const int slot = (gi == 0 && P.periodic[0]) ? nx : gi;
@@ -561,7 +568,7 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
         fc.collide(Famb, hdt, fw0, fw1, fw2, out);
         static_for<10>([&](auto K) {
           constexpr int k = decltype(K)::value;
-          Cx[k][r - 1][c] = out[k];
+          Cx[k][ic(r - 1, c)] = out[k];
         });
       }
     }
@@ -572,14 +579,14 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
         constexpr int ip = decltype(I)::value;
         constexpr int cx = CX[ip], cy = CY[ip];
         if constexpr (ip == 0) {
-          fc.template put<0>(Real(0.5) * (A[0][r - 1][c + 1] + A[0][r][c + 1]));
+          fc.template put<0>(Real(0.5) * (A[0][ia(r - 1, c + 1)] + A[0][ia(r, c + 1)]));
         } else {
           const Real w = yn(cy + 1);  // fraction toward the upper cell
-          Real lo = A[ip][r - 1][c + 1], hi = A[ip][r][c + 1];
+          Real lo = A[ip][ia(r - 1, c + 1)], hi = A[ip][ia(r, c + 1)];
           if constexpr (cx != 0) {
             const Real wx = xt(cx < 0 ? 0 : 1);
-            lo = lerp_(lo, A[ip][r - 1][c + 1 - cx], wx);
-            hi = lerp_(hi, A[ip][r][c + 1 - cx], wx);
+            lo = lerp_(lo, A[ip][ia(r - 1, c + 1 - cx)], wx);
+            hi = lerp_(hi, A[ip][ia(r, c + 1 - cx)], wx);
           }
           fc.template put<ip>(lerp_(lo, hi, w));
         }
@@ -592,7 +599,7 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
         static_for<10>([&](auto K) {
           constexpr int k = decltype(K)::value;
           constexpr int ip = YD[k];
-          mu[k] = Real(CY[ip]) * (uR * Lam[ip][r][c + 1] - uL * Lam[ip][r - 1][c + 1]);
+          mu[k] = Real(CY[ip]) * (uR * Lam[ip][ia(r, c + 1)] - uL * Lam[ip][ia(r - 1, c + 1)]);
         });
         fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, lw, hw, lam);
         const int slot = (gj == 0 && P.periodic[1]) ? ny : gj;
@@ -602,7 +609,7 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
         fc.collide(Famb, hdt, fw0, fw1, fw2, out);
         static_for<10>([&](auto K) {
           constexpr int k = decltype(K)::value;
-          Dy[k][r - 1][c] = out[k];
+          Dy[k][id(r - 1, c)] = out[k];
         });
       }
     }
@@ -637,8 +644,8 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
           constexpr int ip = decltype(I)::value;
           Real v = fbc[ip] * Real(4.0 / 3.0) - fo[ip] * Real(1.0 / 3.0);
           constexpr int kx = cross_index<0>(ip), ky = cross_index<1>(ip), kz = cross_index<2>(ip);
-          if constexpr (kx >= 0) v -= Real(CX[ip]) * updx * (Cx[kx][r - 1][c + 1] - Cx[kx][r - 1][c]);
-          if constexpr (ky >= 0) v -= Real(CY[ip]) * updy * (Dy[ky][r][c] - Dy[ky][r - 1][c]);
+          if constexpr (kx >= 0) v -= Real(CX[ip]) * updx * (Cx[kx][ic(r - 1, c + 1)] - Cx[kx][ic(r - 1, c)]);
+          if constexpr (ky >= 0) v -= Real(CY[ip]) * updy * (Dy[ky][id(r, c)] - Dy[ky][id(r - 1, c)]);
           if constexpr (kz >= 0) {
             v -= Real(CZ[ip]) * updz * (zp[kz] - zm[kz]);
             zm[kz] = zp[kz];
@@ -679,7 +686,7 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
 
   if (!ADJ && partial) {
     // fixed-order block reduction of the flow sums
-    double* red = reinterpret_cast<double*>(&A[0][0][0]);
+    double* red = reinterpret_cast<double*>(&A[0][ia(0, 0)]);
     // every column of the tile has the same z width (z is uniform here)
     red[t] = interior ? flow * (P.wd[0][mi] * P.wd[1][mj] * P.wd[2][1]) : 0.0;
     __syncthreads();
