@@ -368,6 +368,7 @@ void dugks_free(dugks_ctx* ctx) {
   for (int b = 0; b < 2; b++) hipFree(c.f[b]);
   hipFree(c.fbx);
   hipFree(c.adj_faces);
+  hipFree(c.stats);
   for (int a = 0; a < 3; a++) {
     hipFree(c.tab[a]);
     hipFree(c.tabf[a]);
@@ -473,14 +474,44 @@ int dugks_get_schedule(const dugks_ctx* ctx, int out[6]) {
   return DUGKS_OK;
 }
 
+int dugks_get_statistics(dugks_ctx* ctx, double* profiles, double diag[12], int* rows) {
+  CHECK_CTX();
+  if (!profiles || !diag) return fail(ctx, DUGKS_EARG, "null output");
+  Ctx& c = ctx->c;
+  const int pa = stats_axis(c.s);
+  const int np = c.s.n[pa];
+  if (rows) *rows = np;
+  const size_t need = stats_scratch_doubles(np);
+  if (c.stats_len < need) {
+    hipFree(c.stats);
+    c.stats = nullptr;
+    c.stats_len = 0;
+    HIPCHK(c.alloc((void**)&c.stats, need * sizeof(double)));
+    c.stats_len = need;
+  }
+  double* out = c.stats + need - ((size_t)np * 9 + 12);
+  HIPCHK(launch_stats(c, c.f[c.cur], pa, c.stats, out));
+  std::vector<double> sums((size_t)np * 9 + 12);
+  HIPCHK(hipMemcpyAsync(sums.data(), out, sums.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  finish_statistics(c.s, sums.data(), profiles, diag);
+  return DUGKS_OK;
+}
+
 int dugks_write_outputs(dugks_ctx* ctx, int step, const char* dir) {
   CHECK_CTX();
-  const size_t N = ctx->c.s.ncell();
+  const Setup& s = ctx->c.s;
+  const size_t N = s.ncell();
+  std::vector<double> prof((size_t)s.n[stats_axis(s)] * 9);
+  double diag[12];
+  int rc = dugks_get_statistics(ctx, prof.data(), diag, nullptr);
+  if (rc) return rc;
   std::vector<double> m(4 * N);
-  int rc = dugks_get_macro(ctx, m.data(), m.data() + N, m.data() + 2 * N, m.data() + 3 * N);
+  rc = dugks_get_macro(ctx, m.data(), m.data() + N, m.data() + 2 * N, m.data() + 3 * N);
   if (rc) return rc;
   try {
-    write_outputs(ctx->c.s, step, m.data(), m.data() + N, m.data() + 2 * N, m.data() + 3 * N, dir ? dir : "");
+    write_outputs(s, step, m.data(), m.data() + N, m.data() + 2 * N, m.data() + 3 * N, prof.data(), diag,
+                  dir ? dir : "");
   } catch (const ConfigError& e) {
     return fail(ctx, e.code, e.what());
   }

@@ -57,6 +57,11 @@ hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled)
 // Adjoint-face pass of the fused kernel (see adjoint.hip); handled = false
 // when the fused kernel does not cover the configuration.
 hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const double* lam, void* faces, bool* handled);
+// Planar statistics along axis pa into out (device): [n_pa][9] row sums of
+// u, v, w, uw, uv, vw, uu, vv, ww, then global sums, min and max of
+// rho, u, v, w.  scratch holds stats_scratch_doubles(n_pa) doubles.
+size_t stats_scratch_doubles(int np);
+hipError_t launch_stats(Ctx& c, const void* f, int pa, double* scratch, double* out);
 hipError_t launch_macro(Ctx& c, const void* f, double* rho, double* ux, double* uy, double* uz);
 hipError_t launch_export(Ctx& c, const void* f, double* dst);
 hipError_t launch_import(Ctx& c, const double* src, void* f);
@@ -85,6 +90,8 @@ struct Ctx {
   void* f[2] = {nullptr, nullptr};
   int cur = 0;                // f[cur] holds the current state
   void* fbx = nullptr;        // ghost-inclusive f-bar (two-pass kernels, adjoint)
+  double* stats = nullptr;      // planar statistics: partial sums and results
+  size_t stats_len = 0;
   void* adj_faces = nullptr;    // adjoint: 19 face cotangents per face, 3 axes, ghost-inclusive, arithmetic type
   double* tab[3] = {nullptr, nullptr, nullptr};
   double* wd[3] = {nullptr, nullptr, nullptr};

@@ -96,72 +96,58 @@ void slice(const Setup& s, const char* tag, int fixed_axis, int step, const doub
 }  // namespace
 
 void write_outputs(const Setup& s, int step, const double* rho, const double* ux,
-                   const double* uy, const double* uz, const std::string& dir) {
+                   const double* uy, const double* uz, const double* profiles, const double diag[12],
+                   const std::string& dir) {
   slice(s, "X", 0, step, rho, ux, uy, uz, dir);
   slice(s, "Y", 1, step, rho, ux, uy, uz, dir);
   slice(s, "Z", 2, step, rho, ux, uy, uz, dir);
-  const int nx = s.n[0], ny = s.n[1], nz = s.n[2];
-  const size_t N = s.ncell();
-  double mean[4] = {0, 0, 0, 0}, mn[4], mx[4];
-  for (int q = 0; q < 4; q++) {
-    mn[q] = 99999;
-    mx[q] = -99999;
-  }
-  const double* F[4] = {rho, ux, uy, uz};
-  for (size_t c = 0; c < N; c++)
-    for (int q = 0; q < 4; q++) {
-      mean[q] += F[q][c];
-      if (F[q][c] < mn[q]) mn[q] = F[q][c];
-      if (F[q][c] > mx[q]) mx[q] = F[q][c];
-    }
-  for (int q = 0; q < 4; q++) mean[q] /= (double)N;
   FILE* file = std::fopen(join_path(dir, "diag.dat").c_str(), "a");
   if (!file) throw ConfigError(DUGKS_EIO, "cannot append diag.dat");
   std::fprintf(file,
                "%8.8d  % 7.7e % 7.7e % 7.7e % 7.7e  % 7.7e % 7.7e % 7.7e % 7.7e  % 7.7e % 7.7e "
                "% 7.7e % 7.7e\n",
-               step, mean[0], mean[1], mean[2], mean[3], mn[0], mn[1], mn[2], mn[3], mx[0], mx[1],
-               mx[2], mx[3]);
+               step, diag[0], diag[1], diag[2], diag[3], diag[4], diag[5], diag[6], diag[7], diag[8], diag[9],
+               diag[10], diag[11]);
   std::fclose(file);
-
   // per wall-normal index: means, Reynolds stresses xz/xy/yz, variances
-  int pa = s.wall_axis();
-  if (pa < 0) pa = 0;
-  const int np = s.n[pa];
-  std::vector<double> acc((size_t)np * 9, 0.0);
-  for (int k = 0; k < nz; k++)
-    for (int j = 0; j < ny; j++)
-      for (int i = 0; i < nx; i++) {
-        const int idx[3] = {i, j, k};
-        const size_t c = ((size_t)k * ny + j) * nx + i;
-        double* a = &acc[(size_t)idx[pa] * 9];
-        a[0] += ux[c];
-        a[1] += uy[c];
-        a[2] += uz[c];
-        a[3] += ux[c] * uz[c];
-        a[4] += ux[c] * uy[c];
-        a[5] += uy[c] * uz[c];
-        a[6] += ux[c] * ux[c];
-        a[7] += uy[c] * uy[c];
-        a[8] += uz[c] * uz[c];
-      }
-  const double cnt = (double)N / np;
+  const int np = s.n[stats_axis(s)];
   file = std::fopen(join_path(dir, "profiles-2.dat").c_str(), "a");
   if (!file) throw ConfigError(DUGKS_EIO, "cannot append profiles-2.dat");
   std::fprintf(file, "%8.8d\n", step);
   for (int m = 0; m < np; m++) {
-    double* a = &acc[(size_t)m * 9];
-    for (int q = 0; q < 9; q++) a[q] /= cnt;
-    a[6] -= a[0] * a[0];
-    a[7] -= a[1] * a[1];
-    a[8] -= a[2] * a[2];
-    a[3] -= a[0] * a[2];
-    a[4] -= a[0] * a[1];
-    a[5] -= a[1] * a[2];
-    std::fprintf(file, "% 7.7e % 7.7e % 7.7e   % 7.7e % 7.7e % 7.7e   % 7.7e % 7.7e % 7.7e \n", a[0],
-                 a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8]);
+    const double* a = profiles + (size_t)m * 9;
+    std::fprintf(file, "% 7.7e % 7.7e % 7.7e   % 7.7e % 7.7e % 7.7e   % 7.7e % 7.7e % 7.7e \n", a[0], a[1],
+                 a[2], a[3], a[4], a[5], a[6], a[7], a[8]);
   }
   std::fclose(file);
+}
+
+int stats_axis(const Setup& s) {
+  const int pa = s.wall_axis();
+  return pa < 0 ? 0 : pa;
+}
+
+void finish_statistics(const Setup& s, const double* sums, double* profiles, double diag[12]) {
+  const int pa = stats_axis(s);
+  const int np = s.n[pa];
+  const double cnt = (double)s.ncell() / np;
+  for (int m = 0; m < np; m++) {
+    const double* a = sums + (size_t)m * 9;
+    double* o = profiles + (size_t)m * 9;
+    for (int q = 0; q < 9; q++) o[q] = a[q] / cnt;
+    o[6] -= o[0] * o[0];
+    o[7] -= o[1] * o[1];
+    o[8] -= o[2] * o[2];
+    o[3] -= o[0] * o[2];
+    o[4] -= o[0] * o[1];
+    o[5] -= o[1] * o[2];
+  }
+  const double* g = sums + (size_t)np * 9;
+  for (int q = 0; q < 4; q++) {
+    diag[q] = g[q] / (double)s.ncell();
+    diag[4 + q] = g[4 + q];
+    diag[8 + q] = g[8 + q];
+  }
 }
 
 }  // namespace dugks

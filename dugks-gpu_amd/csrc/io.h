@@ -18,9 +18,19 @@ void write_dump(const std::string& path, const double* f, size_t count, int real
 void read_dump(const std::string& path, double* f, size_t count, int real_bytes);
 
 // save_tecplot_serial (src/output.cu:152-218) and statistics_serial
-// (src/output.cu:223-334) with FP64 accumulators.  Profiles are taken along
-// the wall-normal axis (x in the reference), averaging over the other two.
+// (src/output.cu:223-334): the slices from the macroscopic fields, the
+// statistics from the device's FP64 sums (finish_statistics).  Profiles are
+// taken along the wall-normal axis (x in the reference), averaging over the
+// other two.
 void write_outputs(const Setup& s, int step, const double* rho, const double* ux,
-                   const double* uy, const double* uz, const std::string& dir);
+                   const double* uy, const double* uz, const double* profiles, const double diag[12],
+                   const std::string& dir);
+// Axis of the planar statistics: the wall axis, else x (statistics_serial
+// averages over the planes normal to x).
+int stats_axis(const Setup& s);
+// From the device sums [n][9] + [4] sums + [4] min + [4] max to the
+// reference's profile columns (means, stresses uw/uv/vw, variances) and
+// diag (means, min, max of rho, u, v, w).
+void finish_statistics(const Setup& s, const double* sums, double* profiles, double diag[12]);
 
 }  // namespace dugks

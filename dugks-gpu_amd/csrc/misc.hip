@@ -265,6 +265,114 @@ __global__ void __launch_bounds__(256) k_axpy(double a, const double* __restrict
   if (t < n) y[t] += a * x[t];
 }
 
+// Planar statistics along axis pa (statistics_serial, src/output.cu:223-334),
+// in FP64 on the device.  Grid (B, n_pa): block (b, m) sums over its share
+// of row m of the u, v, w, uw, uv, vw, uu, vv, ww products (the profile
+// sums) and of rho, u, v, w (the global means), and takes min/max of
+// rho, u, v, w.  Fixed grid, fixed-order reductions: deterministic.
+constexpr int kStatBlocks = 8;  // blocks per row
+constexpr int kStatSums = 13;
+
+template <class Store>
+__global__ void __launch_bounds__(kRedThreads) k_stats(KParams P, const Store* __restrict__ f, int pa,
+                                                       double* __restrict__ part, double* __restrict__ mm) {
+  const int m = blockIdx.y, b = blockIdx.x;
+  const int a1 = pa == 0 ? 1 : 0, a2 = pa == 2 ? 1 : 2;  // the other two axes
+  const long long plane = (long long)P.n[a1] * P.n[a2];
+  double acc[kStatSums];
+  double mn[4], mx[4];
+  for (int q = 0; q < kStatSums; q++) acc[q] = 0;
+  for (int q = 0; q < 4; q++) {
+    mn[q] = 1e300;
+    mx[q] = -1e300;
+  }
+  double F[3];
+  ambient_d(P, F);
+  for (long long s = (long long)b * kRedThreads + threadIdx.x; s < plane; s += (long long)kStatBlocks * kRedThreads) {
+    int idx[3];
+    idx[pa] = m;
+    idx[a1] = (int)(s % P.n[a1]);
+    idx[a2] = (int)(s / P.n[a1]);
+    const long long c = ((long long)idx[2] * P.n[1] + idx[1]) * P.n[0] + idx[0];
+    double r = 0, j[3] = {0, 0, 0};
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      const double x = Codec<Store>::template load<double>(f, ip * P.N + c, P, ip);
+      r += x;
+      j[0] += CX[ip] * x;
+      j[1] += CY[ip] * x;
+      j[2] += CZ[ip] * x;
+    });
+    const double u = j[0] / r + F[0] * P.hdt, v = j[1] / r + F[1] * P.hdt, w = j[2] / r + F[2] * P.hdt;
+    const double val[kStatSums] = {u, v, w, u * w, u * v, v * w, u * u, v * v, w * w, r, u, v, w};
+    for (int q = 0; q < kStatSums; q++) acc[q] += val[q];
+    const double g[4] = {r, u, v, w};
+    for (int q = 0; q < 4; q++) {
+      mn[q] = fmin(mn[q], g[q]);
+      mx[q] = fmax(mx[q], g[q]);
+    }
+  }
+  __shared__ double sh[kStatSums + 8][kRedThreads];
+  for (int q = 0; q < kStatSums; q++) sh[q][threadIdx.x] = acc[q];
+  for (int q = 0; q < 4; q++) {
+    sh[kStatSums + q][threadIdx.x] = mn[q];
+    sh[kStatSums + 4 + q][threadIdx.x] = mx[q];
+  }
+  __syncthreads();
+  for (int st = kRedThreads / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      for (int q = 0; q < kStatSums; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + st];
+      for (int q = 0; q < 4; q++) {
+        sh[kStatSums + q][threadIdx.x] = fmin(sh[kStatSums + q][threadIdx.x], sh[kStatSums + q][threadIdx.x + st]);
+        sh[kStatSums + 4 + q][threadIdx.x] =
+            fmax(sh[kStatSums + 4 + q][threadIdx.x], sh[kStatSums + 4 + q][threadIdx.x + st]);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const long long slot = (long long)m * kStatBlocks + b;
+    for (int q = 0; q < kStatSums; q++) part[slot * kStatSums + q] = sh[q][0];
+    for (int q = 0; q < 8; q++) mm[slot * 8 + q] = sh[kStatSums + q][0];
+  }
+}
+
+// Per row: the profile sums; globally: rho, u, v, w sums, min, max.  out:
+// [np][9] row sums, then [4] global sums, [4] min, [4] max.
+__global__ void __launch_bounds__(kRedThreads) k_stats_finish(const double* __restrict__ part,
+                                                              const double* __restrict__ mm, int np,
+                                                              double* __restrict__ out) {
+  for (int m = threadIdx.x; m < np; m += kRedThreads)
+    for (int q = 0; q < 9; q++) {
+      double a = 0;
+      for (int b = 0; b < kStatBlocks; b++) a += part[((long long)m * kStatBlocks + b) * kStatSums + q];
+      out[(long long)m * 9 + q] = a;
+    }
+  if (threadIdx.x < 4) {
+    const int q = threadIdx.x;
+    double a = 0, lo = 1e300, hi = -1e300;
+    for (long long s = 0; s < (long long)np * kStatBlocks; s++) {
+      a += part[s * kStatSums + 9 + q];
+      lo = fmin(lo, mm[s * 8 + q]);
+      hi = fmax(hi, mm[s * 8 + 4 + q]);
+    }
+    out[(long long)np * 9 + q] = a;
+    out[(long long)np * 9 + 4 + q] = lo;
+    out[(long long)np * 9 + 8 + q] = hi;
+  }
+}
+
+template <class Store>
+hipError_t stats_t(Ctx& c, const void* f, int pa, double* scratch, double* out) {
+  const int np = c.kp.n[pa];
+  double* part = scratch;
+  double* mm = scratch + (size_t)np * kStatBlocks * kStatSums;
+  hipLaunchKernelGGL((k_stats<Store>), dim3(kStatBlocks, np), dim3(kRedThreads), 0, c.stream, c.kp,
+                     static_cast<const Store*>(f), pa, part, mm);
+  hipLaunchKernelGGL(k_stats_finish, dim3(1), dim3(kRedThreads), 0, c.stream, part, mm, np, out);
+  return hipGetLastError();
+}
+
 template <class Store>
 hipError_t macro_t(Ctx& c, const void* f, double* rho, double* ux, double* uy, double* uz) {
   const unsigned nb = (unsigned)((c.kp.N + 255) / 256);
@@ -316,6 +424,14 @@ hipError_t loss_t(Ctx& c, const void* f, int kind, double* value, double* grad, 
 }
 
 }  // namespace
+
+size_t stats_scratch_doubles(int np) { return (size_t)np * kStatBlocks * (kStatSums + 8) + (size_t)np * 9 + 12; }
+
+hipError_t launch_stats(Ctx& c, const void* f, int pa, double* scratch, double* out) {
+  if (c.precision == 0) return stats_t<double>(c, f, pa, scratch, out);
+  if (c.precision == 1) return stats_t<float>(c, f, pa, scratch, out);
+  return stats_t<__half>(c, f, pa, scratch, out);
+}
 
 hipError_t launch_macro(Ctx& c, const void* f, double* rho, double* ux, double* uy, double* uz) {
   if (c.precision == 0) return macro_t<double>(c, f, rho, ux, uy, uz);

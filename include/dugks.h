@@ -110,6 +110,15 @@ int dugks_get_schedule(const dugks_ctx* ctx, int out[6]);
  * src/output.cu:223-334, FP64 accumulators) in dir (NULL = current). */
 int dugks_write_outputs(dugks_ctx* ctx, int step, const char* dir);
 
+/* Planar statistics of the current state along the wall-normal axis (x when
+ * there is no wall), computed on the device with FP64 sums: statistics_serial,
+ * src/output.cu:223-334.  profiles [n][9] with n = *rows (the wall-normal
+ * size; the buffer must hold max(nx, ny, nz) * 9 doubles): mean u, v, w;
+ * stresses <u'w'>, <u'v'>, <v'w'>; variances <u'u'>, <v'v'>, <w'w'> (the
+ * profiles-2.dat columns).  diag[12]: mean, min, max of rho, ux, uy, uz (the
+ * diag.dat columns).  Host buffers; synchronises the stream. */
+int dugks_get_statistics(dugks_ctx* ctx, double* profiles, double diag[12], int* rows);
+
 /* Cell widths of one axis (n values); centres are cumulative midpoints from
  * the low wall/edge at coordinate 0. */
 int dugks_get_spacing(const dugks_ctx* ctx, int axis, double* widths);

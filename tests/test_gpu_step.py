@@ -297,3 +297,26 @@ def test_perturb_forcing_matches_oracle(rng, kernel):
         eng.step(1, istep0=step)
         ora.step(1)
     assert _relerr(eng.get_f(), ora.get_f()) <= 1e-13
+
+
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_FP32], ids=["fp64", "fp32"])
+def test_device_statistics_match_fields(rng, precision):
+    # statistics_serial (src/output.cu:223-334) on the device in FP64 against
+    # the same sums taken in numpy from the macroscopic fields: profiles
+    # along the wall axis (y here) and the global mean / min / max
+    text = configs.turbulent_channel(nx=32, ny=24, nz=8, precision="fp64", flow_rate=0.0)
+    eng = _engine(text, precision=precision, kernel=KERNEL_FUSED)
+    eng.step(3)
+    prof, diag = eng.statistics()
+    rho, ux, uy, uz = eng.macro()  # [k][j][i]
+    assert prof.shape == (24, 9)
+    mean = lambda a: a.mean(axis=(0, 2))
+    U, V, Wv = mean(ux), mean(uy), mean(uz)
+    expect = np.stack([U, V, Wv, mean(ux * uz) - U * Wv, mean(ux * uy) - U * V, mean(uy * uz) - V * Wv,
+                       mean(ux * ux) - U * U, mean(uy * uy) - V * V, mean(uz * uz) - Wv * Wv], axis=1)
+    scale = np.abs(expect).max(axis=0)
+    assert np.all(np.abs(prof - expect) <= 1e-10 * scale + 1e-18)
+    fields = (rho, ux, uy, uz)
+    assert np.allclose(diag[0:4], [a.mean() for a in fields], rtol=1e-12, atol=1e-15)
+    assert np.array_equal(diag[4:8], [a.min() for a in fields])
+    assert np.array_equal(diag[8:12], [a.max() for a in fields])
