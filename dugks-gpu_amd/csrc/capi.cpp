@@ -871,6 +871,16 @@ int dugks_dump_write(const char* path, const double* f, long long count, int rb)
   });
 }
 
+int dugks_velmean(const char* profiles_path, int n, int ignore_before, int mean_col, int stress_col,
+                  const char* out_path) {
+  if (!profiles_path || !out_path || n <= 0 || mean_col < 0 || mean_col > 2 || stress_col < 3 || stress_col > 5)
+    return DUGKS_EARG;
+  return host_guard([&]() {
+    velmean(profiles_path, n, ignore_before, mean_col, stress_col, out_path);
+    return DUGKS_OK;
+  });
+}
+
 int dugks_dump_read(const char* path, double* f, long long count, int rb) {
   if (!path || !f || count < 0) return DUGKS_EARG;
   return host_guard([&]() {

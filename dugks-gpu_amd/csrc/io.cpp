@@ -150,4 +150,52 @@ void finish_statistics(const Setup& s, const double* sums, double* profiles, dou
   }
 }
 
+// Time average and half-channel symmetrisation of profiles-2.dat into
+// VELMEAN.dat (examples/channel/ProcessProfiles.py:17-79 of the reference):
+// the blocks whose step exceeds ignore_before are averaged; the mean and the
+// rms columns are symmetrised as (a + reverse a) / 2, the shear stress as
+// (s - reverse s) / 2.  Output rows: index (1-based), mean, stress, and the
+// three variance columns.
+void velmean(const std::string& in_path, int n, int ignore_before, int mean_col, int stress_col,
+             const std::string& out_path) {
+  FILE* f = std::fopen(in_path.c_str(), "r");
+  if (!f) throw ConfigError(DUGKS_EIO, "cannot open " + in_path);
+  std::vector<double> mean(n, 0.0), stress(n, 0.0), var[3];
+  for (auto& v : var) v.assign(n, 0.0);
+  int count = 0, step = 0;
+  while (std::fscanf(f, "%d", &step) == 1) {
+    const bool use = step > ignore_before;
+    if (use) count++;
+    for (int k = 0; k < n; k++) {
+      double row[9];
+      for (double& x : row)
+        if (std::fscanf(f, "%lf", &x) != 1) {
+          std::fclose(f);
+          throw ConfigError(DUGKS_EIO, "truncated block in " + in_path);
+        }
+      if (use) {
+        mean[k] += row[mean_col];
+        stress[k] += row[stress_col];
+        for (int q = 0; q < 3; q++) var[q][k] += row[6 + q];
+      }
+    }
+  }
+  std::fclose(f);
+  if (count == 0) throw ConfigError(DUGKS_EIO, "no profile block after the ignored steps in " + in_path);
+  auto sym = [&](std::vector<double>& a, double sign) {
+    std::vector<double> r(n);
+    for (int k = 0; k < n; k++) r[k] = 0.5 * (a[k] / count + sign * a[n - 1 - k] / count);
+    a = r;
+  };
+  sym(mean, 1.0);
+  sym(stress, -1.0);
+  for (auto& v : var) sym(v, 1.0);
+  FILE* g = std::fopen(out_path.c_str(), "w");
+  if (!g) throw ConfigError(DUGKS_EIO, "cannot write " + out_path);
+  for (int k = 0; k < n; k++)
+    std::fprintf(g, "%.1f %.17g %.17g %.17g %.17g %.17g\n", (double)(k + 1), mean[k], stress[k], var[0][k],
+                 var[1][k], var[2][k]);
+  std::fclose(g);
+}
+
 }  // namespace dugks

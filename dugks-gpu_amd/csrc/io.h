@@ -33,4 +33,8 @@ int stats_axis(const Setup& s);
 // diag (means, min, max of rho, u, v, w).
 void finish_statistics(const Setup& s, const double* sums, double* profiles, double diag[12]);
 
+// VELMEAN.dat from profiles-2.dat (examples/channel/ProcessProfiles.py:17-79).
+void velmean(const std::string& in_path, int n, int ignore_before, int mean_col, int stress_col,
+             const std::string& out_path);
+
 }  // namespace dugks

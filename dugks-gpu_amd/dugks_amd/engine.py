@@ -268,3 +268,10 @@ def dump_read(path, count, real_bytes=8):
     out = np.empty(int(count))
     _host_check(_lib.load().dugks_dump_read(_b(path), _ptr(out), int(count), int(real_bytes)))
     return out
+
+
+def velmean(profiles_path, n, out_path, ignore_before=0, mean_col=0, stress_col=4):
+    """VELMEAN.dat from profiles-2.dat (the reference's ProcessProfiles.py):
+    time average of the blocks after ignore_before, halves merged."""
+    _host_check(_lib.load().dugks_velmean(_b(profiles_path), int(n), int(ignore_before), int(mean_col),
+                                          int(stress_col), _b(out_path)))

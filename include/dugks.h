@@ -212,6 +212,17 @@ int dugks_mesh_tanh(int n, double alpha, double width, double* faces);
 int dugks_dump_write(const char* path, const double* f, long long count, int real_bytes);
 int dugks_dump_read(const char* path, double* f, long long count, int real_bytes);
 
+/* Post-processing of profiles-2.dat into VELMEAN.dat, the reference's
+ * examples/channel/ProcessProfiles.py:17-79 (host only): the profile blocks
+ * of n rows whose step exceeds ignore_before are averaged, then the two
+ * channel halves are merged -- mean and variances symmetric, the shear
+ * stress antisymmetric.  mean_col / stress_col select the profiles-2.dat
+ * columns of the streamwise mean (0-2: u, v, w) and of the shear stress
+ * (3: <u'w'>, 4: <u'v'>, 5: <v'w'>).  Rows written: index (1-based), mean,
+ * stress, and the three variances. */
+int dugks_velmean(const char* profiles_path, int n, int ignore_before, int mean_col, int stress_col,
+                  const char* out_path);
+
 #ifdef __cplusplus
 }
 #endif

@@ -200,3 +200,31 @@ def test_driver_reports_bad_config(tmp_path):
     r = subprocess.run([exe, str(tmp_path / "missing.txt")], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1
     assert "cannot open config file" in r.stderr
+
+
+def test_velmean_postprocessing(tmp_path):
+    # ProcessProfiles.py (examples/channel/ProcessProfiles.py:17-79): blocks
+    # up to ignore_before are skipped, the rest averaged, the halves merged
+    # (mean and variances even, shear stress odd about the channel centre)
+    n = 6
+    rng = np.random.default_rng(5)
+    blocks = {step: rng.standard_normal((n, 9)) for step in (100, 200, 300, 400)}
+    path = tmp_path / "profiles-2.dat"
+    with open(path, "w") as f:
+        for step, b in blocks.items():
+            f.write(f"{step:08d}\n")
+            for row in b:
+                f.write(" ".join(f"{x: .17e}" for x in row) + "\n")
+    out = tmp_path / "VELMEAN.dat"
+    dugks_amd.velmean(str(path), n, str(out), ignore_before=150, mean_col=0, stress_col=4)
+    got = np.loadtxt(out)
+    avg = np.mean([blocks[s] for s in (200, 300, 400)], axis=0)
+    assert np.array_equal(got[:, 0], np.arange(1, n + 1))
+    assert np.allclose(got[:, 1], 0.5 * (avg[:, 0] + avg[::-1, 0]), rtol=1e-14, atol=1e-15)
+    assert np.allclose(got[:, 2], 0.5 * (avg[:, 4] - avg[::-1, 4]), rtol=1e-14, atol=1e-15)
+    for q in range(3):
+        assert np.allclose(got[:, 3 + q], 0.5 * (avg[:, 6 + q] + avg[::-1, 6 + q]), rtol=1e-14, atol=1e-15)
+    # symmetry of the merged profiles
+    assert np.allclose(got[:, 1], got[::-1, 1]) and np.allclose(got[:, 2], -got[::-1, 2])
+    with pytest.raises(dugks_amd.DugksError):
+        dugks_amd.velmean(str(path), n, str(out), ignore_before=1000)
