@@ -255,6 +255,79 @@ def study_channel(dugks_amd, configs, torch, np, steps):
     return out
 
 
+def study_adjoint(dugks_amd, configs, torch, np, T, ndirs=32):
+    """Adjoint workload: transitional channel 64x64x32 (seeded perturbed
+    laminar state, Re = 2000), J = time-mean wall shear stress over T steps,
+    control G ~ N(0, 1e-6) per cell (seeded).  sqrt(T)-checkpointed
+    gradient in FP64 and FP32, against central differences of J along
+    `ndirs` seeded random directions (eps per precision), FP32 against FP64,
+    and the timing of the sweep against T forward steps."""
+    out = {"workload": "transitional channel 64x64x32, Re=2000, wall-shear loss", "T": T, "directions": ndirs}
+    grads, fds = {}, {}
+    for m, eps in (("fp64", 1.0), ("fp32", 100.0)):
+        prec = {"fp64": dugks_amd.PREC_FP64, "fp32": dugks_amd.PREC_FP32}[m]
+        eng = dugks_amd.Engine(configs.adjoint_channel(precision=m), precision=prec, device=0,
+                               stream=torch.cuda.current_stream().cuda_stream)
+        nx, ny, nz = eng.shape
+        f0 = torch.empty((19, nz, ny, nx), dtype=torch.float64, device="cuda")
+        eng.get_f_device(f0)
+        G = 1e-6 * np.random.default_rng(2024).standard_normal((3, nz, ny, nx))
+        Gd = torch.from_numpy(G).cuda()
+        gd = torch.empty_like(Gd)
+
+        def run_J(ctrl):
+            eng.set_f_device(f0)
+            eng.set_cell_force_device(ctrl)
+            tot = 0.0
+            for n in range(T):
+                eng.step(1, istep0=1 + n)
+                tot += eng.loss(dugks_amd.LOSS_WALL_SHEAR)
+            return tot / T
+
+        eng.set_f_device(f0)
+        eng.set_cell_force_device(Gd)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        J, peak = eng.forward_adjoint(T, gd)
+        torch.cuda.synchronize()
+        t_sweep = time.perf_counter() - t
+        eng.set_f_device(f0)
+        eng.set_cell_force_device(Gd)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        eng.step(T, istep0=1)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t
+        grad = gd.cpu().numpy()
+        grads[m] = grad
+        rng = np.random.default_rng(7)
+        errs = []
+        fds[m] = []
+        for d in range(ndirs):
+            v = 1e-6 * rng.standard_normal(G.shape)
+            vd = torch.from_numpy(v).cuda()
+            jp = run_J(Gd + eps * vd)
+            jm = run_J(Gd - eps * vd)
+            fd = (jp - jm) / (2 * eps)
+            an = float((grad * v).sum())
+            errs.append(abs(fd - an) / abs(an))
+            fds[m].append((fd, v))
+        out[m] = {"J": J, "eps_fd": eps, "fd_rel_err_max": float(max(errs)),
+                  "fd_rel_err_median": float(np.median(errs)), "sweep_s": round(t_sweep, 3),
+                  "forward_s": round(t_fwd, 4), "sweep_over_forward": round(t_sweep / t_fwd, 2),
+                  "sweep_device_bytes": peak}
+        log(f"adjoint study {m}: FD max rel err {max(errs):.2e}, sweep {t_sweep:.2f} s "
+            f"({t_sweep / t_fwd:.1f}x forward)")
+        eng.close()
+    g64, g32 = grads["fp64"], grads["fp32"]
+    out["fp32_vs_fp64_grad_max_rel"] = float(np.abs(g32 - g64).max() / np.abs(g64).max())
+    # the FP32 gradient against the FP64 central differences (the FP32 forward
+    # is too noisy for its own differences at this perturbation size)
+    out["fp32_grad_vs_fp64_fd_max_rel"] = float(max(abs(fd - float((g32 * v).sum())) / abs(fd)
+                                                    for fd, v in fds["fp64"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -265,8 +338,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
-    ap.add_argument("--adj-steps", type=int, default=100)
-    ap.add_argument("--study", choices=["tgv", "channel"],
+    ap.add_argument("--adj-steps", type=int, default=1000,
+                    help="steps of the forward+adjoint sweep on the large channel (0: skip)")
+    ap.add_argument("--study", choices=["tgv", "channel", "adjoint"],
                     help="instead of the benchmark: the FP32 / FP16-store error study on the "
                          "Taylor-Green vortex or the 128^3 channel (prints one JSON line)")
     ap.add_argument("--study-steps", type=int, default=0, help="steps of the study (default 1000 / 10000)")
@@ -294,6 +368,8 @@ def main():
         import numpy as np
         if args.study == "tgv":
             res = study_tgv(dugks_amd, configs, torch, np, args.study_steps or 1000)
+        elif args.study == "adjoint":
+            res = study_adjoint(dugks_amd, configs, torch, np, args.study_steps or 400)
         else:
             res = study_channel(dugks_amd, configs, torch, np, args.study_steps or 10000)
         print(json.dumps({"study": args.study, **res}), flush=True)
