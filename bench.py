@@ -167,29 +167,33 @@ def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32"):
     return res
 
 
-def cpu_baseline(configs, threads):
+def cpu_baseline(configs, threads, target_s=15.0):
     """FP64 oracle (the reference algorithm restated in C++/OpenMP) on a
-    z-slab of the same workload: 256 x 192 x 16 cells, same mesh and dt."""
+    z-slab of the same workload: 256 x 192 x 32 cells, same mesh and dt,
+    stepped in batches of 5 until about target_s seconds of CPU work."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import dugks_amd
     from pyoracle import Oracle
-    text = configs.turbulent_channel(nx=256, ny=192, nz=16)
+    nzs = 32
+    text = configs.turbulent_channel(nx=256, ny=192, nz=nzs)
     dims, p = dugks_amd.derive(text)
     w = dugks_amd.config_spacing(text, 1)
     f0 = dugks_amd.initial_state(text)
-    o = Oracle(dims, (1, 0, 1), (4 * math.pi, 2.0, 4 * math.pi / 3 * 16 / 256), dt=p["dt"],
+    o = Oracle(dims, (1, 0, 1), (4 * math.pi, 2.0, 4 * math.pi / 3 * nzs / 256), dt=p["dt"],
                visc=p["visc"], famb=((180 * 3.0e-5) ** 2, 0, 0), spacing=(None, w, None),
                threads=threads)
     o.set_f(f0)
     o.step(1)  # warm-up
-    n = 3
+    n = 0
     t = time.perf_counter()
-    o.step(n)
+    while time.perf_counter() - t < target_s and n < 400:
+        o.step(5)
+        n += 5
     dt = time.perf_counter() - t
     cells = int(np.prod(dims))
     return dict(value=cells * n / dt / 1e6, unit="MLUPS", cores=threads, kind="port",
-                sample=f"FP64 oracle, {n} steps of a 256x192x16 z-slab of the workload "
+                sample=f"FP64 oracle, {n} steps of a 256x192x{nzs} z-slab of the workload "
                        f"({cells} cells, same stretched mesh and dt), {dt:.1f} s")
 
 
