@@ -255,23 +255,23 @@ __global__ void __launch_bounds__(256) k_face_ghosts(KParams P, Real* __restrict
 }
 
 // ---- kernel 3: one thread per interior cell ----
+constexpr int kCX = 32, kCY = 8, kCZ = 1;  // cell tile of a 256-thread block
 template <class Real, class Store>
 __global__ void __launch_bounds__(256) k_adj_cells(KParams P, const Store* __restrict__ f,
                                                    const double* __restrict__ lam, const Real* __restrict__ faces,
                                                    double* __restrict__ lam_prev, double* __restrict__ grad,
                                                    int wall_axis) {
-  // 32 x 8 cell tiles (a wave covers 32 x 2 cells, so y-neighbour faces are
-  // shared inside a wave), dealt to the XCDs in contiguous runs of tiles
-  // (x fastest) so that neighbouring tiles share an L2: each face value is
-  // gathered by cells of up to four tiles
-  const int gx = (P.n[0] + 31) / 32, gy = (P.n[1] + 7) / 8;
-  const int ntiles = gx * gy * P.n[2];
+  // kCX x kCY x kCZ cell tiles, dealt to the XCDs in contiguous runs of
+  // tiles (x fastest) so that neighbouring tiles share an L2: each face
+  // value is gathered by cells of up to eight tiles
+  const int gx = (P.n[0] + kCX - 1) / kCX, gy = (P.n[1] + kCY - 1) / kCY, gz = (P.n[2] + kCZ - 1) / kCZ;
+  const int ntiles = gx * gy * gz;
   const int b = blockIdx.x;
   const int tt = (ntiles % 8 == 0) ? (b % 8) * (ntiles / 8) + b / 8 : b;
-  const int ci = (tt % gx) * 32 + (int)threadIdx.x % 32;
-  const int cj = ((tt / gx) % gy) * 8 + (int)threadIdx.x / 32;
-  const int ck = tt / (gx * gy);
-  if (ci >= P.n[0] || cj >= P.n[1]) return;
+  const int ci = (tt % gx) * kCX + (int)threadIdx.x % kCX;
+  const int cj = ((tt / gx) % gy) * kCY + ((int)threadIdx.x / kCX) % kCY;
+  const int ck = (tt / (gx * gy)) * kCZ + (int)threadIdx.x / (kCX * kCY);
+  if (ci >= P.n[0] || cj >= P.n[1] || ck >= P.n[2]) return;
   const long long c = ((long long)ck * P.n[1] + cj) * P.n[0] + ci;
   const int t[3] = {ci + 1, cj + 1, ck + 1};
   Real lfb[Q];
@@ -357,7 +357,8 @@ hipError_t run_adjoint(Ctx& cx, const void* f, const double* lam_next, double* l
     hipLaunchKernelGGL((k_face_ghosts<Real>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, cx.stream, P, faces,
                        b);
   }
-  const unsigned nbc = (unsigned)((P.n[0] + 31) / 32 * ((P.n[1] + 7) / 8) * P.n[2]);
+  const unsigned nbc = (unsigned)((P.n[0] + kCX - 1) / kCX * ((P.n[1] + kCY - 1) / kCY) *
+                                  ((P.n[2] + kCZ - 1) / kCZ));
   hipLaunchKernelGGL((k_adj_cells<Real, Store>), dim3(nbc), dim3(256), 0, cx.stream, P,
                      static_cast<const Store*>(f), lam_next, static_cast<const Real*>(cx.adj_faces), lam_prev, grad,
                      wall);

@@ -188,6 +188,9 @@ __global__ void __launch_bounds__(256) k_perturb(KParams P, int step, double xwi
 //           off each wall, rho u_s = sum_i (c_is + h F_s) f_i (linear in f);
 //           J = sum over both wall planes / (2 * cells per plane).
 //   kind 1: kinetic energy J = sum 0.5 rho |u|^2 dV / V, u = j/rho + h F.
+// The wall-shear loss only involves the two wall planes: unless the gradient
+// array is being initialised (grad set, gacc == 0), the kernel visits those
+// planes only (2 nx nz of the N cells on a y-wall channel).
 template <class Store>
 __global__ void __launch_bounds__(kRedThreads) k_loss(KParams P, const Store* __restrict__ f, int kind,
                                                       int aw, int as, double visc, double vol,
@@ -196,10 +199,24 @@ __global__ void __launch_bounds__(kRedThreads) k_loss(KParams P, const Store* __
   double acc[1] = {0};
   double F[3];
   ambient_d(P, F);
-  for (long long c = (long long)blockIdx.x * kRedThreads + threadIdx.x; c < P.N;
-       c += (long long)kRedBlocks * kRedThreads) {
+  const bool wall_only = kind == 0 && !(grad && gacc == 0.0);
+  const int b1 = (aw + 1) % 3, b2 = (aw + 2) % 3;
+  const long long npl = (long long)P.n[b1] * P.n[b2];
+  const long long count = wall_only ? (P.n[aw] > 1 ? 2 : 1) * npl : P.N;
+  for (long long w = (long long)blockIdx.x * kRedThreads + threadIdx.x; w < count;
+       w += (long long)kRedBlocks * kRedThreads) {
     int idx[3];
-    cell_coords(P, c, idx[0], idx[1], idx[2]);
+    long long c = w;
+    if (wall_only) {
+      // w -> (side, in-plane position) -> cell of the low or high wall plane
+      const long long q = w % npl;
+      idx[aw] = w < npl ? 0 : P.n[aw] - 1;
+      idx[b1] = (int)(q % P.n[b1]);
+      idx[b2] = (int)(q / P.n[b1]);
+      c = ((long long)idx[2] * P.n[1] + idx[1]) * P.n[0] + idx[0];
+    } else {
+      cell_coords(P, c, idx[0], idx[1], idx[2]);
+    }
     double x[Q];
     static_for<Q>([&](auto I) {
       constexpr int ip = decltype(I)::value;
