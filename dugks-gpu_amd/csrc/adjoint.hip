@@ -377,6 +377,9 @@ hipError_t launch_adjoint(Ctx& c, const void* f, const double* lam_next, double*
       wall = a;
     }
   if (walls > 1) return hipErrorNotSupported;
+  bool handled = false;
+  hipError_t e1 = launch_adjoint_onepass(c, f, lam_next, lam_prev, grad_force, &handled);
+  if (e1 != hipSuccess || handled) return e1;
   const size_t rb = c.precision == 0 ? 8 : 4;
   if (!c.adj_faces) {
     const size_t bytes = (size_t)c.kp.eN * 3 * Q * rb;

@@ -708,7 +708,18 @@ int dugks_adjoint_step(dugks_ctx* ctx, const double* lam_next, double* lam_prev,
   CHECK_CTX();
   if (!lam_next || !lam_prev) return fail(ctx, DUGKS_EARG, "null adjoint buffer");
   Ctx& c = ctx->c;
+  const size_t bytes = (size_t)Q * c.s.ncell() * sizeof(double);
+  const char *a = reinterpret_cast<const char*>(lam_next), *b = reinterpret_cast<const char*>(lam_prev);
+  if (a < b + bytes && b < a + bytes) return fail(ctx, DUGKS_EARG, "lam_prev overlaps lam_next");
   HIPCHK(launch_adjoint(c, c.f[c.cur], lam_next, lam_prev, grad_force));
+  return DUGKS_OK;
+}
+
+int dugks_set_adjoint_path(dugks_ctx* ctx, int path) {
+  CHECK_CTX();
+  if (path != DUGKS_ADJOINT_AUTO && path != DUGKS_ADJOINT_SPLIT && path != DUGKS_ADJOINT_ONEPASS)
+    return fail(ctx, DUGKS_EARG, "bad adjoint path");
+  ctx->c.adjoint_path = path;
   return DUGKS_OK;
 }
 

@@ -70,6 +70,10 @@ hipError_t launch_flow_control(Ctx& c, const void* f);
 hipError_t launch_perturb_force(Ctx& c, int step);
 hipError_t launch_adjoint(Ctx& c, const void* f, const double* lam_next, double* lam_prev,
                           double* grad_force);
+// One-pass adjoint step (adjoint_fused.hip); handled = false when it does not
+// cover the configuration.
+hipError_t launch_adjoint_onepass(Ctx& c, const void* f, const double* lam_next, double* lam_prev,
+                                  double* grad_force, bool* handled);
 // Loss value into dev_value (added to it when value_acc) and, when dev_grad
 // is set, its gradient dJ/df written into dev_grad (grad_acc == 0) or added
 // to it times grad_acc.
@@ -86,6 +90,7 @@ struct Ctx {
   bool own_stream = false;
   int precision = 0;
   int kernel = 0;
+  int adjoint_path = 0;       // DUGKS_ADJOINT_AUTO, _SPLIT or _ONEPASS
   size_t store_bytes = 8;     // bytes per stored f value
   void* f[2] = {nullptr, nullptr};
   int cur = 0;                // f[cur] holds the current state

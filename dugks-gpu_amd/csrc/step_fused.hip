@@ -27,16 +27,7 @@ namespace dugks {
 
 namespace {
 
-// populations with a non-zero component along each axis
-constexpr int XD[10] = {1, 2, 7, 8, 9, 10, 11, 12, 13, 14};
-constexpr int YD[10] = {3, 4, 7, 8, 9, 10, 15, 16, 17, 18};
-constexpr int ZD[10] = {5, 6, 11, 12, 13, 14, 15, 16, 17, 18};
-constexpr int ZP[5] = {5, 11, 12, 15, 16};  // c_z = +1
-
-template <int A>
-__device__ constexpr int dirs(int k) {
-  return A == 0 ? XD[k] : (A == 1 ? YD[k] : ZD[k]);
-}
+#include "fused_common.h"
 
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
 
@@ -47,212 +38,6 @@ struct Tile {
   static constexpr int NT = (NROW + NHALO + 63) / 64 * 64;
   static constexpr int AC = TX + 2;
   static_assert(TX % 16 == 0 && TX <= 64, "rows of 16, 32, 48 or 64 lanes");
-};
-
-template <class Real>
-__device__ __forceinline__ Real lerp_(Real a, Real b, Real m) {
-  return (Real(1) - m) * a + m * b;
-}
-
-template <class Real>
-__device__ __forceinline__ void ambient_force(const KParams& P, Real F[3]) {
-  F[0] = (Real)P.famb_dev[0];
-  F[1] = (Real)P.famb_dev[1];
-  F[2] = (Real)P.famb_dev[2];
-}
-
-// f-bar+ of interior cell (i, j, k) (0-based).
-template <class Real, class Store>
-__device__ __forceinline__ void fbar_cell(const KParams& P, const StateBuf<Store>& f, unsigned Nb, int i, int j,
-                                          int k, const Real Famb[3], Real (&fb)[Q]) {
-  const unsigned c = ((unsigned)k * P.n[1] + j) * P.n[0] + i;
-  Real ff[Q];
-  static_for<Q>([&](auto I) {
-    constexpr int ip = decltype(I)::value;
-    ff[ip] = ld_state<Real, Store>(f, c * (unsigned)sizeof(Store), ip * Nb, P, ip);
-  });
-  Real F[3] = {Famb[0], Famb[1], Famb[2]};
-  if (P.fcell) {
-    F[0] += (Real)P.fcell[c];
-    F[1] += (Real)P.fcell[P.N + c];
-    F[2] += (Real)P.fcell[2 * P.N + c];
-  }
-  centre_fbar<Real>(ff, F, (Real)P.hdt, (Real)P.bw0, (Real)P.bw1, (Real)P.bw2, fb);
-}
-
-// f-bar+ of a column that may be a wall ghost in x and/or y: the ghost value
-// is the linear extrapolation q f(1) + (1-q) f(2) from the wall side.
-// gi, gj are 0-based; a wall ghost is -1 or n; periodic indices are wrapped
-// by the caller.
-template <class Real, class Store>
-__device__ void fbar_column(const KParams& P, const StateBuf<Store>& f, unsigned Nb, int gi, int gj, int k,
-                            const Real Famb[3], Real (&fb)[Q]) {
-  const bool gx = gi < 0 || gi >= P.n[0];
-  const bool gy = gj < 0 || gj >= P.n[1];
-  if (!gx && !gy) {
-    fbar_cell<Real, Store>(P, f, Nb, gi, gj, k, Famb, fb);
-    return;
-  }
-  int xs[2] = {gi, gi}, ys[2] = {gj, gj};
-  Real wx[2] = {1, 0}, wy[2] = {1, 0};
-  if (gx) {
-    const bool lo = gi < 0;
-    const Real q = (Real)(lo ? P.qlo[0] : P.qhi[0]);
-    xs[0] = lo ? 0 : P.n[0] - 1;
-    xs[1] = lo ? 1 : P.n[0] - 2;
-    wx[0] = q;
-    wx[1] = Real(1) - q;
-  }
-  if (gy) {
-    const bool lo = gj < 0;
-    const Real q = (Real)(lo ? P.qlo[1] : P.qhi[1]);
-    ys[0] = lo ? 0 : P.n[1] - 1;
-    ys[1] = lo ? 1 : P.n[1] - 2;
-    wy[0] = q;
-    wy[1] = Real(1) - q;
-  }
-  // tensor-product extrapolation (the x -> y ghost fill of the reference
-  // gives the same linear combination)
-  bool first = true;
-  for (int b = 0; b < (gy ? 2 : 1); b++)
-    for (int a = 0; a < (gx ? 2 : 1); a++) {
-      const Real w = wx[a] * wy[b];
-      Real t[Q];
-      fbar_cell<Real, Store>(P, f, Nb, xs[a], ys[b], k, Famb, t);
-      static_for<Q>([&](auto I) {
-        constexpr int ip = decltype(I)::value;
-        fb[ip] = first ? w * t[ip] : fb[ip] + w * t[ip];
-      });
-      first = false;
-    }
-}
-
-// Index of population ip in the crossing set of axis A (-1 if it does not
-// cross a face normal to A).
-template <int A>
-__device__ constexpr int cross_index(int ip) {
-  for (int k = 0; k < 10; k++)
-    if (dirs<A>(k) == ip) return k;
-  return -1;
-}
-
-// Reconstructed populations of one face normal to axis A, held as the ten
-// values that cross the face plus the partial moments of the nine that do
-// not (those only enter rho and j of the face collision).
-template <class Real, int A>
-struct Face {
-  Real cross[10];
-  Real rho = 0, jx = 0, jy = 0, jz = 0;
-  template <int ip>
-  __device__ __forceinline__ void put(Real v) {
-    constexpr int k = cross_index<A>(ip);
-    if constexpr (k >= 0) {
-      cross[k] = v;
-    } else {
-      rho += v;
-      if constexpr (CX[ip] != 0) jx += CX[ip] > 0 ? v : -v;
-      if constexpr (CY[ip] != 0) jy += CY[ip] > 0 ? v : -v;
-      if constexpr (CZ[ip] != 0) jz += CZ[ip] > 0 ? v : -v;
-    }
-  }
-  // Half-way bounce-back: populations entering the domain through a wall
-  // take the value of their opposite (src/dugks.cu:807-823).
-  __device__ __forceinline__ void bounce(bool low_wall, bool high_wall) {
-    if (low_wall) {
-      static_for<10>([&](auto K) {
-        constexpr int k = decltype(K)::value;
-        constexpr int ip = dirs<A>(k);
-        if constexpr (cv(A, ip) > 0) cross[k] = cross[cross_index<A>(OPP[ip])];
-      });
-    }
-    if (high_wall) {
-      static_for<10>([&](auto K) {
-        constexpr int k = decltype(K)::value;
-        constexpr int ip = dirs<A>(k);
-        if constexpr (cv(A, ip) < 0) cross[k] = cross[cross_index<A>(OPP[ip])];
-      });
-    }
-  }
-  // Face collision (f2ruvwh + feq_s, src/dugks.cu:675-709, 1201-1207):
-  // rho = sum fbar, u = j/rho + (h/2) F, out = fw1 fbar + (fw2 + fw0 (c-u).F/RT) feq
-  // for the crossing populations.
-  __device__ __forceinline__ void collide(const Real F[3], Real hdt, Real fw0, Real fw1, Real fw2,
-                                          Real (&out)[10]) {
-    static_for<10>([&](auto K) {
-      constexpr int k = decltype(K)::value;
-      constexpr int ip = dirs<A>(k);
-      const Real v = cross[k];
-      rho += v;
-      if constexpr (CX[ip] != 0) jx += CX[ip] > 0 ? v : -v;
-      if constexpr (CY[ip] != 0) jy += CY[ip] > 0 ? v : -v;
-      if constexpr (CZ[ip] != 0) jz += CZ[ip] > 0 ? v : -v;
-    });
-    const Real inv = rcp_real(rho);
-    const Real hh = Real(0.5) * hdt;
-    const Real ux = jx * inv + hh * F[0];
-    const Real uy = jy * inv + hh * F[1];
-    const Real uz = jz * inv + hh * F[2];
-    const Real uu = ux * ux + uy * uy + uz * uz;
-    const Real base = Real(1) - Real(1.5) * uu;
-    const Real uF = ux * F[0] + uy * F[1] + uz * F[2];
-    const Real P0 = fw2 - Real(3) * fw0 * uF;
-    static_for<10>([&](auto K) {
-      constexpr int k = decltype(K)::value;
-      constexpr int ip = dirs<A>(k);
-      const Real cu = Real(CX[ip]) * ux + Real(CY[ip]) * uy + Real(CZ[ip]) * uz;
-      const Real cF = Real(CX[ip]) * F[0] + Real(CY[ip]) * F[1] + Real(CZ[ip]) * F[2];
-      const Real feq = Real(wgt(ip)) * rho * (base + Real(3) * cu + Real(4.5) * cu * cu);
-      out[k] = fw1 * cross[k] + (P0 + Real(3) * fw0 * cF) * feq;
-    });
-  }
-  // Transpose of collide (after bounce) and bounce: given the cotangent mu
-  // of the ten outputs, the cotangent of the 19 reconstructed values before
-  // bounce-back.  d out_q / d fbar_p = fw1 delta_qp + (d/d rho + c_p.d/d j)
-  // of the equilibrium term, so lam_p = fw1 mu_p + A + B.c_p; a population
-  // that entered through a wall took its opposite's value, so its cotangent
-  // moves to that opposite.
-  __device__ __forceinline__ void adjoint(const Real F[3], Real hdt, Real fw0, Real fw1, Real fw2,
-                                          const Real (&mu)[10], bool low_wall, bool high_wall, Real (&lam)[Q]) {
-    static_for<10>([&](auto K) {
-      constexpr int k = decltype(K)::value;
-      constexpr int ip = dirs<A>(k);
-      const Real v = cross[k];
-      rho += v;
-      if constexpr (CX[ip] != 0) jx += CX[ip] > 0 ? v : -v;
-      if constexpr (CY[ip] != 0) jy += CY[ip] > 0 ? v : -v;
-      if constexpr (CZ[ip] != 0) jz += CZ[ip] > 0 ? v : -v;
-    });
-    const Real inv = rcp_real(rho);
-    const Real hh = Real(0.5) * hdt;
-    const Real ut[3] = {jx * inv, jy * inv, jz * inv};
-    const Real u[3] = {ut[0] + hh * F[0], ut[1] + hh * F[1], ut[2] + hh * F[2]};
-    Real lq[Q];
-    static_for<Q>([&](auto I) {
-      constexpr int ip = decltype(I)::value;
-      constexpr int k = cross_index<A>(ip);
-      if constexpr (k >= 0) lq[ip] = mu[k]; else lq[ip] = Real(0);
-    });
-    Real g_rho, g_u[3], g_F[3];
-    feq_adjoint<Real>(lq, fw2, fw0, rho, u, F, g_rho, g_u, g_F, Crossing<A>{});
-    const Real Ac = g_rho - (g_u[0] * ut[0] + g_u[1] * ut[1] + g_u[2] * ut[2]) * inv;
-    const Real Bx = g_u[0] * inv, By = g_u[1] * inv, Bz = g_u[2] * inv;
-    Real post[Q];
-    static_for<Q>([&](auto I) {
-      constexpr int ip = decltype(I)::value;
-      post[ip] = fw1 * lq[ip] + Ac + Bx * Real(CX[ip]) + By * Real(CY[ip]) + Bz * Real(CZ[ip]);
-    });
-    static_for<Q>([&](auto I) {
-      constexpr int ip = decltype(I)::value;
-      Real v = post[ip];
-      if constexpr (cv(A, ip) != 0) {
-        const bool in = (low_wall && cv(A, ip) > 0) || (high_wall && cv(A, ip) < 0);
-        const bool opp_in = (low_wall && cv(A, ip) < 0) || (high_wall && cv(A, ip) > 0);
-        if (in) v = Real(0);
-        if (opp_in) v += post[OPP[ip]];
-      }
-      lam[ip] = v;
-    });
-  }
 };
 
 // Inputs and outputs of the adjoint-face mode of the fused kernel.

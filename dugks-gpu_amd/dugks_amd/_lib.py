@@ -16,6 +16,7 @@ ERRORS = {
 }
 PREC_CONFIG, PREC_FP64, PREC_FP32, PREC_FP16S = -1, 0, 1, 2
 KERNEL_AUTO, KERNEL_TWOPASS, KERNEL_FUSED = -1, 0, 1
+ADJOINT_AUTO, ADJOINT_SPLIT, ADJOINT_ONEPASS = 0, 1, 2
 LOSS_WALL_SHEAR, LOSS_KINETIC_ENERGY = 0, 1
 
 
@@ -68,6 +69,7 @@ PROTOTYPES = {
     "dugks_read_dump": (_i, [_ctxp, _i, _cp, _i]),
     "dugks_global_sums": (_i, [_ctxp, _dp]),
     "dugks_adjoint_step": (_i, [_ctxp, _vp, _vp, _vp]),
+    "dugks_set_adjoint_path": (_i, [_ctxp, _i]),
     "dugks_loss": (_i, [_ctxp, _i, _dp, _vp]),
     "dugks_forward_adjoint": (_i, [_ctxp, _i, _i, _i, _i, _dp, _vp, _vp, _dp]),
     # host-only helpers (no device needed)

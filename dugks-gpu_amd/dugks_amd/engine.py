@@ -203,6 +203,13 @@ class Engine:
         self._check(self._lib.dugks_loss(self._h, int(kind), ctypes.byref(v), _dev_ptr(grad)))
         return v.value
 
+    def set_adjoint_path(self, path):
+        """ADJOINT_AUTO or ADJOINT_SPLIT: the split path (adjoint-face pass +
+        per-cell gather); ADJOINT_ONEPASS: the one-pass adjoint kernel where
+        it applies (FP32 arithmetic, 16 x 12 tiles, at most one wall axis),
+        else the split path."""
+        self._check(self._lib.dugks_set_adjoint_path(self._h, int(path)))
+
     def adjoint_step(self, lam_next, lam_prev, grad_force=None):
         self._check(self._lib.dugks_adjoint_step(self._h, _dev_ptr(lam_next), _dev_ptr(lam_prev),
                                                  _dev_ptr(grad_force)))

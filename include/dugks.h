@@ -169,10 +169,21 @@ int dugks_global_sums(dugks_ctx* ctx, double out[8]);
 /* Discrete adjoint of one step about the context's current state f^n:
  * lam_prev = (df^{n+1}/df^n)^T lam_next and, when grad_force is not NULL,
  * grad_force += (df^{n+1}/dF_cell)^T lam_next.  All three are device arrays:
- * lam_* SoA [19][nz][ny][nx] FP64, grad_force [3][nz][ny][nx] FP64.  The
- * state is not advanced. */
+ * lam_* SoA [19][nz][ny][nx] FP64, grad_force [3][nz][ny][nx] FP64;
+ * lam_prev must not overlap lam_next.  The state is not advanced. */
 int dugks_adjoint_step(dugks_ctx* ctx, const double* lam_next, double* lam_prev,
                        double* grad_force);
+
+/* Adjoint kernel path.  DUGKS_ADJOINT_AUTO (the default) and
+ * DUGKS_ADJOINT_SPLIT run the split path (adjoint-face pass, then a per-cell
+ * gather), the faster one on MI355X.  DUGKS_ADJOINT_ONEPASS runs the one-pass
+ * kernel where it applies (FP32 arithmetic, 16 x 12 fused-step tiles, at most
+ * one wall axis) and the split path elsewhere.  lam_prev must not overlap
+ * lam_next on any path. */
+#define DUGKS_ADJOINT_AUTO 0
+#define DUGKS_ADJOINT_SPLIT 1
+#define DUGKS_ADJOINT_ONEPASS 2
+int dugks_set_adjoint_path(dugks_ctx* ctx, int path);
 
 /* Loss functionals J(f) = sum_cells g(f) evaluated on the current state and
  * their gradient dJ/df (device SoA FP64, written, not accumulated).

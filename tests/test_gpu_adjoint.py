@@ -12,7 +12,7 @@ import pytest
 
 from conftest import C, random_state
 import dugks_amd
-from dugks_amd import KERNEL_FUSED, KERNEL_TWOPASS, PREC_FP32, PREC_FP64, configs
+from dugks_amd import KERNEL_FUSED, KERNEL_TWOPASS, PREC_FP16S, PREC_FP32, PREC_FP64, configs
 from pyoracle import Oracle
 
 torch = pytest.importorskip("torch")
@@ -57,8 +57,9 @@ CASES = {
     "tall": dict(nx=32, ny=24, nz=10, periods_y=0, mesh_y="tanh:0.85", dt_rule="min", ambient_fx=2e-4,
                  xwidth=3.2, ywidth=2.0, zwidth=1.0),
     "tall_walls_x": dict(nx=16, ny=12, nz=6, periods_x=0, ambient_fz=2e-4, xwidth=1.6, ywidth=1.2, zwidth=0.6),
+    "tall_periodic": dict(nx=32, ny=24, nz=6, xwidth=3.2, ywidth=2.4, zwidth=0.6, ambient_fx=1e-4),
 }
-FUSED_CASES = ("periodic", "walls_y", "walls_x", "tiles", "tall", "tall_walls_x")
+FUSED_CASES = ("periodic", "walls_y", "walls_x", "tiles", "tall", "tall_walls_x", "tall_periodic")
 
 
 @pytest.mark.parametrize("case", list(CASES))
@@ -90,6 +91,78 @@ def test_adjoint_step_matches_oracle(rng, case, kernel, precision, tol):
     g = np.moveaxis(g_d.cpu().numpy(), 0, -1)
     assert np.abs(lp - lp_ref).max() <= tol * np.abs(lp_ref).max()
     assert np.abs(g - g_ref).max() <= tol * np.abs(g_ref).max()
+
+
+# configurations of the one-pass adjoint kernel (FP32 arithmetic, 16 x 12
+# tiles): several tiles per axis, interior tiles away from the walls, and
+# several z-chunks (their boundary planes are recomputed by both chunks)
+ONEPASS_CASES = {
+    "channel": dict(nx=48, ny=36, nz=40, periods_y=0, mesh_y="tanh:0.85", dt_rule="min", ambient_fx=2e-4,
+                    xwidth=4.8, ywidth=2.0, zwidth=4.0),
+    "walls_x": dict(nx=48, ny=24, nz=20, periods_x=0, mesh_x="tanh:0.8", dt_rule="min", ambient_fz=2e-4,
+                    xwidth=4.8, ywidth=2.4, zwidth=2.0),
+    "periodic": dict(nx=32, ny=24, nz=8, xwidth=3.2, ywidth=2.4, zwidth=0.8, ambient_fx=1e-4),
+}
+
+
+@pytest.mark.parametrize("case", list(ONEPASS_CASES))
+@pytest.mark.parametrize("precision", [PREC_FP32, PREC_FP16S], ids=["fp32", "fp16"])
+def test_onepass_adjoint_matches_split_path(rng, case, precision):
+    text = configs.base(visc=0.02, **ONEPASS_CASES[case])
+    eng = _engine(text, precision=precision)
+    eng.set_f(random_state(eng.shape, rng))
+    nx, ny, nz = eng.shape
+    eng.set_cell_force(*(1e-4 * rng.standard_normal((3, nz, ny, nx))))
+    dev = torch.device("cuda")
+    lam = torch.from_numpy(rng.standard_normal((19, nz, ny, nx))).to(dev)
+    out = {}
+    for path in (dugks_amd.ADJOINT_SPLIT, dugks_amd.ADJOINT_ONEPASS):
+        eng.set_adjoint_path(path)
+        lp = torch.empty_like(lam)
+        g = torch.full((3, nz, ny, nx), 0.5, dtype=torch.float64, device=dev)  # the gradient is added to
+        eng.adjoint_step(lam, lp, g)
+        torch.cuda.synchronize()
+        out[path] = (lp.cpu().numpy(), g.cpu().numpy() - 0.5)
+    (l0, g0), (l1, g1) = out[dugks_amd.ADJOINT_SPLIT], out[dugks_amd.ADJOINT_ONEPASS]
+    # the one-pass kernel ran: its FP32 sums run in another order
+    assert not np.array_equal(l0, l1)
+    assert np.abs(l1 - l0).max() <= 2e-5 * np.abs(l0).max()
+    assert np.abs(g1 - g0).max() <= 2e-5 * np.abs(g0).max()
+
+
+@pytest.mark.parametrize("case", ["tall", "tall_walls_x", "tall_periodic"])
+def test_onepass_adjoint_matches_oracle(rng, case):
+    text = configs.base(visc=0.02, **CASES[case])
+    eng = _engine(text, precision=PREC_FP32)
+    eng.set_adjoint_path(dugks_amd.ADJOINT_ONEPASS)
+    ora, famb = _oracle(eng, text)
+    eng.set_f(random_state(eng.shape, rng))
+    f0 = eng.get_f()
+    nx, ny, nz = eng.shape
+    G = 1e-4 * rng.standard_normal((3, nz, ny, nx))
+    eng.set_cell_force(*G)
+    ora.set_f(f0)
+    ora.set_cell_force(*(G + famb.reshape(3, 1, 1, 1)))
+    lam = rng.standard_normal(f0.shape)
+    lp_ref, g_ref = ora.adjoint_step(lam)
+    dev = torch.device("cuda")
+    lam_d = torch.from_numpy(_soa(lam)).to(dev)
+    lp_d = torch.empty_like(lam_d)
+    g_d = torch.zeros((3, nz, ny, nx), dtype=torch.float64, device=dev)
+    eng.adjoint_step(lam_d, lp_d, g_d)
+    torch.cuda.synchronize()
+    lp = _aos(lp_d.cpu().numpy())
+    g = np.moveaxis(g_d.cpu().numpy(), 0, -1)
+    assert np.abs(lp - lp_ref).max() <= 2e-5 * np.abs(lp_ref).max()
+    assert np.abs(g - g_ref).max() <= 2e-5 * np.abs(g_ref).max()
+
+
+def test_adjoint_step_rejects_overlapping_buffers(rng):
+    eng = _engine(configs.base(visc=0.02, **ONEPASS_CASES["periodic"]), precision=PREC_FP32)
+    nx, ny, nz = eng.shape
+    lam = torch.zeros((19, nz, ny, nx), dtype=torch.float64, device=torch.device("cuda"))
+    with pytest.raises(dugks_amd.DugksError):
+        eng.adjoint_step(lam, lam)
 
 
 def _wall_shear_grad(eng, famb, nu):
@@ -178,10 +251,13 @@ def test_forward_adjoint_matches_central_differences(rng):
         assert abs(fd - an) <= 1e-6 * abs(an), (trial, fd, an)
 
 
-def test_fp32_gradient_against_fp64(rng):
-    e64, f64, G, T = _adjoint_problem(rng, PREC_FP64)
+@pytest.mark.parametrize("ny,path", [(16, 0), (24, 2)], ids=["split", "onepass"])
+def test_fp32_gradient_against_fp64(rng, ny, path):
+    # ny = 24 gives 16 x 12 tiles, where the one-pass adjoint applies
+    e64, f64, G, T = _adjoint_problem(rng, PREC_FP64, ny=ny)
     J64, g64, _ = _grad(e64, f64, G, T)
-    e32, f32, _, _ = _adjoint_problem(rng, PREC_FP32)
+    e32, f32, _, _ = _adjoint_problem(rng, PREC_FP32, ny=ny)
+    e32.set_adjoint_path(path)
     J32, g32, _ = _grad(e32, f32, G, T)
     assert abs(J32 - J64) <= 1e-5 * abs(J64)
     assert np.abs(g32 - g64).max() <= 1e-3 * np.abs(g64).max()
