@@ -111,7 +111,7 @@ def run_mode(dugks_amd, configs, torch, precision, args, np=None):
     return res, stats
 
 
-def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32"):
+def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32", path="split"):
     """Forward + adjoint over T steps of the same 256x192x256 channel with a
     constant driving force (the adjoint does not differentiate the flow-rate
     controller): wall-shear loss, sqrt(T) checkpoints in HBM, gradient with
@@ -120,6 +120,7 @@ def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32"):
     prec = {"fp64": dugks_amd.PREC_FP64, "fp32": dugks_amd.PREC_FP32}[precision]
     text = configs.large_channel(precision=precision, flow_rate=0)
     eng = dugks_amd.Engine(text, precision=prec, device=0, stream=torch.cuda.current_stream().cuda_stream)
+    eng.set_adjoint_path({"split": dugks_amd.ADJOINT_SPLIT, "onepass": dugks_amd.ADJOINT_ONEPASS}[path])
     nz, ny, nx = eng.shape[2], eng.shape[1], eng.shape[0]
     f0 = torch.empty((19, nz, ny, nx), dtype=torch.float64, device="cuda")
     eng.get_f_device(f0)
@@ -154,7 +155,7 @@ def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32"):
     ok = math.isfinite(J) and math.isfinite(gmax)
     eng.close()
     fwd_step = t_fwd / T
-    res = dict(precision=precision, T=T, interval=int(math.ceil(math.sqrt(T))), loss="wall shear",
+    res = dict(precision=precision, path=path, T=T, interval=int(math.ceil(math.sqrt(T))), loss="wall shear",
                forward_ms_per_step=round(fwd_step * 1e3, 4),
                adjoint_step_ms=round(t_adj * 1e3, 4),
                adjoint_over_forward_step=round(t_adj / fwd_step, 3),
@@ -344,6 +345,8 @@ def main():
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--adj-steps", type=int, default=1000,
                     help="steps of the forward+adjoint sweep on the large channel (0: skip)")
+    ap.add_argument("--adj-path", default="split", choices=["split", "onepass"],
+                    help="adjoint kernels of the adjoint leg (the split path is the default)")
     ap.add_argument("--study", choices=["tgv", "channel", "adjoint"],
                     help="instead of the benchmark: the FP32 / FP16-store error study on the "
                          "Taylor-Green vortex or the 128^3 channel (prints one JSON line)")
@@ -397,7 +400,7 @@ def main():
     stats.clear()
     adjoint = None
     if args.adj_steps > 0:
-        adjoint = adjoint_leg(dugks_amd, configs, torch, args.adj_steps)
+        adjoint = adjoint_leg(dugks_amd, configs, torch, args.adj_steps, path=args.adj_path)
     primary = "fp32" if "fp32" in results else modes[0]
     r = results[primary]
     cpu = None

@@ -55,3 +55,16 @@ def random_state(shape, rng, amp=0.05, drho=0.02):
 @pytest.fixture
 def rng():
     return np.random.default_rng(1234)
+
+
+def reflect_state(f, axis):
+    """Mirror x_axis -> L - x_axis of a state f (nz, ny, nx, 19): the cells
+    reverse along the axis and each population takes the slot of the one
+    whose velocity has that component negated."""
+    cn = C.copy()
+    cn[:, axis] *= -1
+    q_new = [int(np.where((C == cn[q]).all(1))[0][0]) for q in range(19)]
+    sp = np.flip(f, axis=2 - axis)
+    out = np.empty_like(sp)
+    out[..., q_new] = sp
+    return out

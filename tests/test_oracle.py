@@ -11,7 +11,7 @@ import math
 import numpy as np
 import pytest
 
-from conftest import C, OPP, RT, W, feq, moments, random_state
+from conftest import C, OPP, RT, W, feq, moments, random_state, reflect_state
 from pyoracle import Oracle
 import numpy_dugks
 
@@ -193,6 +193,40 @@ def test_wall_axis_permutation_equivariance(rng):
     b.set_f(_rotate_state(f0, perm))
     b.step(3)
     assert np.abs(_rotate_state(a.get_f(), perm) - b.get_f()).max() < 1e-15
+
+
+# (mirrored axis, periodic flags, spacing of the mirrored axis or None,
+# force): mirror-symmetric meshes only; the mirrored force component changes
+# sign
+REFLECT_CASES = {
+    "periodic_x": (0, (1, 1, 1), None, (1e-4, 2e-4, -3e-4)),
+    "periodic_z": (2, (1, 1, 1), None, (1e-4, 2e-4, -3e-4)),
+    "walls_y_stretched": (1, (1, 0, 1), np.array([0.05, 0.12, 0.33, 0.33, 0.12, 0.05]), (2e-4, 0.0, 1e-4)),
+    "walls_x_stretched": (0, (0, 1, 1), np.array([0.1, 0.2, 0.3, 0.3, 0.2, 0.1]), (0.0, 1e-4, 2e-4)),
+}
+
+
+@pytest.mark.parametrize("case", list(REFLECT_CASES))
+def test_mirror_equivariance(rng, case):
+    # step(R f) = R step(f) for a mirror R of a lattice axis: pins the sign
+    # and side of every +/-c weight (stretched tables included), the wall
+    # ghosts and bounce-back at both walls, and the force terms
+    axis, per, d, F = REFLECT_CASES[case]
+    shape = (6, 6, 6)
+    spacing = [None, None, None]
+    if d is not None:
+        spacing[axis] = d / d.sum()
+    F = np.array(F)
+    Fm = F.copy()
+    Fm[axis] *= -1
+    f0 = random_state(shape, rng)
+    a = Oracle(shape, per, (1, 1, 1), dt=0.02, visc=0.02, famb=F, spacing=spacing)
+    a.set_f(f0)
+    a.step(3)
+    b = Oracle(shape, per, (1, 1, 1), dt=0.02, visc=0.02, famb=Fm, spacing=spacing)
+    b.set_f(reflect_state(f0, axis))
+    b.step(3)
+    assert np.abs(reflect_state(a.get_f(), axis) - b.get_f()).max() < 1e-14
 
 
 @pytest.mark.parametrize("stretched", [False, True])
