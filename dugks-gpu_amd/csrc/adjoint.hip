@@ -15,9 +15,13 @@
 //      is lam_fbar_q = fw1 lam_ft_q + A + B.c_q; after the transpose of the
 //      wall bounce-back the face stores the cotangent of its 19
 //      reconstructed values (in the arithmetic type);
-//   3. k_adj_cells: per cell, S3^T as a gather of those values over the
-//      faces whose stencil contains the cell (the wall ghost extrapolation
-//      folded in), then S5^T and S1^T.
+//   3. the gather: S3^T of those values over the faces whose stencil
+//      contains each cell (the wall ghost extrapolation folded in), then S5^T
+//      and S1^T.  k_adj_cells_z (FP32 arithmetic, 32 x 8 column tiles)
+//      stages each plane of face cotangents in LDS and marches along z;
+//      k_adj_cells (every other case) runs one thread per cell over face
+//      arrays with periodic ghost copies (k_face_ghosts).
+// The fused step's tiles also have a one-pass form (adjoint_fused.hip).
 // Deterministic (no atomics).  At most one wall axis.
 #include "device_common.h"
 
@@ -331,6 +335,311 @@ __global__ void __launch_bounds__(256) k_adj_cells(KParams P, const Store* __res
   }
 }
 
+// ---- kernel 3, z-marching form (FP32 arithmetic, 32 x 8 column tiles) ----
+// A workgroup owns a kGX x kGY tile of cell columns and marches along z over
+// a chunk of planes.  Per plane it stages the x-, y- and z-face cotangents of
+// the tile and a one-deep ring ((kGY+2) x (kGX+2) ghost-inclusive positions,
+// periodic indices wrapped, so no ghost copies are needed) in LDS, with the
+// next plane's loads in flight in registers; each cell then gathers its
+// transposed in-plane lerps from LDS.  The transposed z-lerps are pushed
+// into the f-bar+ cotangents of planes p-1, p and p+1 held in registers:
+// the x- and y-faces of plane p (z-lerped toward p - c_z in the step) and
+// the z-face p+1/2 (normal lerp between p and p+1).  Wall ghosts are folded
+// at the in-plane level: a cell beside a wall adds q (or 1 - q) times the
+// in-plane gather at the ghost position, which takes the same z transposes
+// as its own.  Each face value is read from HBM about 1.3 times (the ring)
+// instead of about 2.5 times.
+constexpr int kGX = 32, kGY = 8;
+constexpr int kGW = kGX + 2;          // staged row length
+constexpr int kGP = (kGY + 2) * kGW;  // staged positions per array and population
+constexpr int kGT = kGX * kGY;        // lanes: one per cell column
+static_assert(kGP <= 2 * kGT, "two staged positions per lane at most");
+
+template <class Real, class Store>
+__global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __restrict__ fin,
+                                                     const double* __restrict__ lam, const Real* __restrict__ faces,
+                                                     double* __restrict__ lam_prev, double* __restrict__ grad,
+                                                     int TZ) {
+  __shared__ Real S[3][Q][kGP];   // face cotangents of plane p: x-faces, y-faces, z-faces p+1/2
+  __shared__ Real WXs[5][kGW];    // per staged column: wR for c = -1, 0, +1, then the shift weights
+  __shared__ Real WYs[5][kGY + 2];  // per staged row
+
+  const int nx = P.n[0], ny = P.n[1], nz = P.n[2];
+  const int t = threadIdx.x;
+  // XCD-aware tile order (workgroup b runs on XCD b % 8)
+  const int gxn = nx / kGX, gyn = ny / kGY;
+  const int ntiles = gxn * gyn * (int)gridDim.z;
+  const int b = blockIdx.x + gxn * (blockIdx.y + gyn * blockIdx.z);
+  const int tt = (ntiles % 8 == 0) ? (b % 8) * (ntiles / 8) + b / 8 : b;
+  const int bx = tt % gxn, by = (tt / gxn) % gyn, bz = tt / (gxn * gyn);
+  const int i0 = bx * kGX, j0 = by * kGY;
+  const int z0 = bz * TZ, zend = min(z0 + TZ, nz);
+  const int ex = P.e[0], ey = P.e[1];
+
+  // the lane's cell, and its staged (ghost-inclusive, tile-local) position
+  const int r = t / kGX, c = t % kGX;
+  const int R = r + 1, Cc = c + 1;
+  const unsigned plane = (unsigned)nx * (unsigned)ny;
+  const unsigned colcell = (unsigned)(j0 + r) * (unsigned)nx + (unsigned)(i0 + c);
+
+  // weights of the staged columns and rows (table_slot maps the periodic ghosts)
+  for (int u = t; u < 5 * kGW + 5 * (kGY + 2); u += kGT) {
+    const bool isx = u < 5 * kGW;
+    const int v = isx ? u : u - 5 * kGW;
+    const int n = isx ? kGW : kGY + 2;
+    const int k = v / n, q = v % n;
+    const int a = isx ? 0 : 1;
+    const int slot = table_slot(P, a, (isx ? i0 : j0) + q);
+    const Real w = tabv<Real>(P, a, slot * 9 + (k < 3 ? k : k + 3));
+    if (isx)
+      WXs[k][q] = w;
+    else
+      WYs[k][q] = w;
+  }
+  // z is periodic and uniform
+  const Real mz6 = tabv<Real>(P, 2, 9 + 6), mz7 = tabv<Real>(P, 2, 9 + 7);
+  const Real wRz[3] = {tabv<Real>(P, 2, 9 + 0), tabv<Real>(P, 2, 9 + 1), tabv<Real>(P, 2, 9 + 2)};
+
+  Real Famb[3];
+  Famb[0] = (Real)P.famb_dev[0];
+  Famb[1] = (Real)P.famb_dev[1];
+  Famb[2] = (Real)P.famb_dev[2];
+  const Real hdt = (Real)P.hdt, bw0 = (Real)P.bw0, bw1 = (Real)P.bw1, bw2 = (Real)P.bw2;
+
+  // staged positions of this lane: t and t + kGT
+  unsigned exy[2];
+  bool okp[2], okx[2], oky[2];
+  for (int h = 0; h < 2; h++) {
+    const int pp = t + h * kGT;
+    okp[h] = pp < kGP;
+    const int Rs = pp / kGW, Cs = pp % kGW;
+    int x = i0 + Cs, y = j0 + Rs;
+    if (P.periodic[0]) x = x == 0 ? nx : (x == nx + 1 ? 1 : x);
+    if (P.periodic[1]) y = y == 0 ? ny : (y == ny + 1 ? 1 : y);
+    exy[h] = (unsigned)y * (unsigned)ex + (unsigned)x;
+    okx[h] = okp[h] && Cs <= kGX;  // x-face slots i0 .. i0+kGX
+    oky[h] = okp[h] && Rs <= kGY;  // y-face slots j0 .. j0+kGY
+  }
+  const unsigned pop_bytes = (unsigned)(P.eN * (long long)sizeof(Real));
+  const RawBuf fx(faces, Q * pop_bytes), fy(faces + Q * P.eN, Q * pop_bytes), fz(faces + 2 * Q * P.eN, Q * pop_bytes);
+  Real pre[2][3][Q];
+  // face cotangents of plane p (ghost-inclusive z slot p + 1) into registers
+  auto stage_load = [&](int p) {
+    int zs = p + 1;
+    zs = zs == 0 ? nz : (zs == nz + 1 ? 1 : zs);
+    const unsigned zoff = (unsigned)zs * (unsigned)ey * (unsigned)ex;
+    for (int h = 0; h < 2; h++) {
+      const unsigned vb = (zoff + exy[h]) * (unsigned)sizeof(Real);
+      const unsigned vx = okx[h] ? vb : kOutOfRange, vy = oky[h] ? vb : kOutOfRange, vz = okp[h] ? vb : kOutOfRange;
+      static_for<Q>([&](auto I) {
+        constexpr int ip = decltype(I)::value;
+        pre[h][0][ip] = fx.ld<Real>(vx, ip * pop_bytes);
+        pre[h][1][ip] = fy.ld<Real>(vy, ip * pop_bytes);
+        pre[h][2][ip] = fz.ld<Real>(vz, ip * pop_bytes);
+      });
+    }
+  };
+  auto stage_store = [&]() {
+    for (int h = 0; h < 2; h++) {
+      if (!okp[h]) continue;
+      const int pp = t + h * kGT;
+      static_for<Q>([&](auto I) {
+        constexpr int ip = decltype(I)::value;
+        S[0][ip][pp] = pre[h][0][ip];
+        S[1][ip][pp] = pre[h][1][ip];
+        S[2][ip][pp] = pre[h][2][ip];
+      });
+    }
+  };
+
+  // In-plane transposed lerps at staged position (Rp, Cp) for population ip:
+  // the cotangent of Z(p) (x- and y-faces) and of Zn(p+1/2) (z-faces).
+  // GUARD: the position is a wall ghost, whose stencil reaches outside the
+  // staged region (nothing is there: 0).
+  auto sget = [&](int A, int ip, int Rp, int Cp, bool guard) -> Real {
+    if (guard && (Rp < 0 || Rp > kGY + 1 || Cp < 0 || Cp > kGX + 1)) return Real(0);
+    return S[A][ip][Rp * kGW + Cp];
+  };
+  auto inplane = [&](auto I, int Rp, int Cp, bool guard, Real& hz_out) -> Real {
+    constexpr int ip = decltype(I)::value;
+    constexpr int cx = CX[ip], cy = CY[ip];
+    // weights at a guarded position's outside neighbours only multiply 0;
+    // clamp their index so that they stay table values
+    auto WX = [&](int k, int Cq) { return WXs[k][guard ? min(max(Cq, 0), kGW - 1) : Cq]; };
+    auto WY = [&](int k, int Rq) { return WYs[k][guard ? min(max(Rq, 0), kGY + 1) : Rq]; };
+    if constexpr (ip == 0) {
+      hz_out = sget(2, 0, Rp, Cp, guard);
+      return Real(0.5) * (sget(0, 0, Rp, Cp, guard) + sget(0, 0, Rp, Cp - 1, guard)) +
+             Real(0.5) * (sget(1, 0, Rp, Cp, guard) + sget(1, 0, Rp - 1, Cp, guard));
+    } else {
+      // x-faces: slot Cp (this column is the left cell), Cp-1 (right cell);
+      // rows Rp and Rp+c_y
+      const Real a0 = Real(1) - WX(cx + 1, Cp), a1 = WX(cx + 1, Cp - 1);
+      Real hx = a0 * sget(0, ip, Rp, Cp, guard) + a1 * sget(0, ip, Rp, Cp - 1, guard);
+      if constexpr (cy != 0) {
+        const Real m0 = Real(1) - WY(cy < 0 ? 3 : 4, Rp), m1 = WY(cy < 0 ? 3 : 4, Rp + cy);
+        hx = m0 * hx + m1 * (a0 * sget(0, ip, Rp + cy, Cp, guard) + a1 * sget(0, ip, Rp + cy, Cp - 1, guard));
+      }
+      // y-faces: slot Rp (this row is the lower cell), Rp-1 (upper cell);
+      // columns Cp and Cp+c_x
+      const Real b0 = Real(1) - WY(cy + 1, Rp), b1 = WY(cy + 1, Rp - 1);
+      Real hy = b0 * sget(1, ip, Rp, Cp, guard) + b1 * sget(1, ip, Rp - 1, Cp, guard);
+      if constexpr (cx != 0) {
+        const Real m0 = Real(1) - WX(cx < 0 ? 3 : 4, Cp), m1 = WX(cx < 0 ? 3 : 4, Cp + cx);
+        hy = m0 * hy + m1 * (b0 * sget(1, ip, Rp, Cp + cx, guard) + b1 * sget(1, ip, Rp - 1, Cp + cx, guard));
+      }
+      // z-face p+1/2: columns Cp, Cp+c_x; rows Rp, Rp+c_y
+      Real hz = sget(2, ip, Rp, Cp, guard);
+      Real hz2 = Real(0);
+      if constexpr (cy != 0) hz2 = sget(2, ip, Rp + cy, Cp, guard);
+      if constexpr (cx != 0) {
+        const Real m0 = Real(1) - WX(cx < 0 ? 3 : 4, Cp), m1 = WX(cx < 0 ? 3 : 4, Cp + cx);
+        hz = m0 * hz + m1 * sget(2, ip, Rp, Cp + cx, guard);
+        if constexpr (cy != 0) hz2 = m0 * hz2 + m1 * sget(2, ip, Rp + cy, Cp + cx, guard);
+      }
+      if constexpr (cy != 0) {
+        const Real m0 = Real(1) - WY(cy < 0 ? 3 : 4, Rp), m1 = WY(cy < 0 ? 3 : 4, Rp + cy);
+        hz = m0 * hz + m1 * hz2;
+      }
+      hz_out = hz;
+      return hx + hy;
+    }
+  };
+
+  // wall ghost folded into this cell: position and weight (at most one wall axis)
+  int gR = -1, gC = -1;
+  Real gq = Real(0);
+  if (!P.periodic[1]) {
+    if (j0 == 0 && R <= 2) {
+      gR = 0, gC = Cc, gq = R == 1 ? (Real)P.qlo[1] : Real(1) - (Real)P.qlo[1];
+    } else if (j0 + kGY == ny && R >= kGY - 1) {
+      gR = kGY + 1, gC = Cc, gq = R == kGY ? (Real)P.qhi[1] : Real(1) - (Real)P.qhi[1];
+    }
+  } else if (!P.periodic[0]) {
+    if (i0 == 0 && Cc <= 2) {
+      gR = R, gC = 0, gq = Cc == 1 ? (Real)P.qlo[0] : Real(1) - (Real)P.qlo[0];
+    } else if (i0 + kGX == nx && Cc >= kGX - 1) {
+      gR = R, gC = kGX + 1, gq = Cc == kGX ? (Real)P.qhi[0] : Real(1) - (Real)P.qhi[0];
+    }
+  }
+  const bool ghost = gR >= 0;
+
+  Real accP[Q], accC[Q];
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    accP[ip] = Real(0);
+    accC[ip] = Real(0);
+  });
+  const StateBuf<Store> bin(fin, Q * (unsigned)(P.N * (long long)sizeof(Store)));
+  const unsigned Nb = (unsigned)(P.N * (long long)sizeof(Store));
+  const RawBuf lamb(lam, (unsigned)(Q * P.N * 8));
+  const unsigned lam_pop = (unsigned)(P.N * 8);
+
+  for (int p = z0 - 1; p <= zend; p++) {
+    stage_load(p);
+    __syncthreads();  // the previous plane's gather is done with S
+    stage_store();
+    __syncthreads();
+    // transposed reconstruction, merged with the shift of the cotangent
+    // window after the plane p-1 is complete
+    Real accN[Q];
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      constexpr int cz = CZ[ip];
+      Real hz;
+      Real hzr = inplane(I, R, Cc, false, hz);
+      if (ghost) {
+        Real gz;
+        const Real g = inplane(I, gR, gC, true, gz);
+        hzr += gq * g;
+        hz += gq * gz;
+      }
+      Real nxt = Real(0);
+      if constexpr (cz == 0) {
+        accC[ip] += hzr;
+      } else if constexpr (cz > 0) {
+        accC[ip] += (Real(1) - mz7) * hzr;
+        accP[ip] += mz7 * hzr;
+      } else {
+        accC[ip] += (Real(1) - mz6) * hzr;
+        nxt = mz6 * hzr;
+      }
+      const Real wn = ip == 0 ? Real(0.5) : wRz[cz + 1];
+      accC[ip] += (Real(1) - wn) * hz;
+      accN[ip] = nxt + wn * hz;
+    });
+    // centre-stage adjoint of plane p-1 (as k_adj_cells)
+    if (p - 1 >= z0) {
+      const unsigned cell = (unsigned)(p - 1) * plane + colcell;
+      const long long cl = (long long)cell;
+      Real ff[Q], lp[Q];
+      static_for<Q>([&](auto I) {
+        constexpr int ip = decltype(I)::value;
+        lp[ip] = (Real)lamb.ld_f64(cell * 8u, ip * lam_pop);
+        ff[ip] = ld_state<Real, Store>(bin, cell * (unsigned)sizeof(Store), ip * Nb, P, ip);
+      });
+      static_for<Q>([&](auto I) {
+        constexpr int ip = decltype(I)::value;
+        accP[ip] += Real(4.0 / 3.0) * lp[ip];
+      });
+      Real rho = 0, j[3] = {0, 0, 0};
+      static_for<Q>([&](auto I) {
+        constexpr int ip = decltype(I)::value;
+        rho += ff[ip];
+        if constexpr (CX[ip] != 0) j[0] += CX[ip] > 0 ? ff[ip] : -ff[ip];
+        if constexpr (CY[ip] != 0) j[1] += CY[ip] > 0 ? ff[ip] : -ff[ip];
+        if constexpr (CZ[ip] != 0) j[2] += CZ[ip] > 0 ? ff[ip] : -ff[ip];
+      });
+      Real F[3], u[3], ut[3];
+      const Real invr = Real(1) / rho;
+      for (int a = 0; a < 3; a++) {
+        F[a] = Famb[a];
+        if (P.fcell) F[a] += (Real)P.fcell[a * P.N + cl];
+        ut[a] = j[a] * invr;
+        u[a] = ut[a] + hdt * F[a];
+      }
+      Real g_rho, g_u[3], g_F[3];
+      feq_adjoint<Real>(accP, bw2, bw0, rho, u, F, g_rho, g_u, g_F, AllPops{});
+      const Real Acoef = g_rho - (g_u[0] * ut[0] + g_u[1] * ut[1] + g_u[2] * ut[2]) * invr;
+      static_for<Q>([&](auto I) {
+        constexpr int ip = decltype(I)::value;
+        const Real v = -lp[ip] * Real(1.0 / 3.0) + bw1 * accP[ip] + Acoef +
+                       (g_u[0] * Real(CX[ip]) + g_u[1] * Real(CY[ip]) + g_u[2] * Real(CZ[ip])) * invr;
+        lam_prev[ip * P.N + cl] = (double)v;
+      });
+      if (grad) {
+        for (int a = 0; a < 3; a++) grad[a * P.N + cl] += (double)(g_F[a] + hdt * g_u[a]);
+      }
+    }
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      accP[ip] = accC[ip];
+      accC[ip] = accN[ip];
+    });
+  }
+}
+
+// The z-marching gather covers FP32 arithmetic on 32 x 8 column tiles with z
+// periodic and uniform, at most one wall axis (x or y), and face arrays that
+// 32-bit buffer offsets reach.
+bool gather_z_supported(const Ctx& c) {
+  const Setup& s = c.s;
+  if (c.precision == 0 || c.kernel == 0) return false;
+  if (s.n[0] % kGX != 0 || s.n[1] % kGY != 0 || !s.periodic[2] || !s.uniform[2]) return false;
+  if (!s.periodic[0] && !s.periodic[1]) return false;
+  if (!s.periodic[0] && s.n[0] < 3) return false;
+  if (!s.periodic[1] && s.n[1] < 3) return false;
+  if ((double)Q * c.kp.eN * 4.0 >= 2147483648.0 || (double)Q * c.kp.N * 8.0 >= 2147483648.0) return false;
+  return true;
+}
+
+int gather_z_chunk(const KParams& P) {
+  const long long cols = (long long)(P.n[0] / kGX) * (P.n[1] / kGY);
+  int tz = 64;
+  while (tz > 8 && cols * ((P.n[2] + tz - 1) / tz) < 4 * 256) tz /= 2;
+  return tz;
+}
+
 template <class Real, class Store>
 hipError_t run_adjoint(Ctx& cx, const void* f, const double* lam_next, double* lam_prev, double* grad, int wall) {
   const KParams& P = cx.kp;
@@ -350,6 +659,15 @@ hipError_t run_adjoint(Ctx& cx, const void* f, const double* lam_next, double* l
     hipLaunchKernelGGL((k_adj_faces<Real, 0>), dim3(nbe), dim3(256), 0, cx.stream, P, fbx, lam_next, faces);
     hipLaunchKernelGGL((k_adj_faces<Real, 1>), dim3(nbe), dim3(256), 0, cx.stream, P, fbx, lam_next, faces);
     hipLaunchKernelGGL((k_adj_faces<Real, 2>), dim3(nbe), dim3(256), 0, cx.stream, P, fbx, lam_next, faces);
+  }
+  if constexpr (!std::is_same<Real, double>::value) {
+    if (gather_z_supported(cx)) {
+      const int TZ = gather_z_chunk(P);
+      dim3 grid(P.n[0] / kGX, P.n[1] / kGY, (P.n[2] + TZ - 1) / TZ);
+      hipLaunchKernelGGL((k_adj_cells_z<Real, Store>), grid, dim3(kGT), 0, cx.stream, P, static_cast<const Store*>(f),
+                         lam_next, static_cast<const Real*>(cx.adj_faces), lam_prev, grad, TZ);
+      return hipGetLastError();
+    }
   }
   for (int b = 0; b < 3; b++) {
     if (!P.periodic[b]) continue;
