@@ -29,10 +29,7 @@ namespace dugks {
 
 namespace {
 
-template <class Real>
-__device__ __forceinline__ Real lerp_(Real a, Real b, Real m) {
-  return (Real(1) - m) * a + m * b;
-}
+#include "fused_common.h"
 
 __device__ __forceinline__ long long ext_index(const KParams& P, int m0, int m1, int m2) {
   return ((long long)m2 * P.e[1] + m1) * P.e[0] + m0;
@@ -420,10 +417,24 @@ __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __r
     okx[h] = okp[h] && Cs <= kGX;  // x-face slots i0 .. i0+kGX
     oky[h] = okp[h] && Rs <= kGY;  // y-face slots j0 .. j0+kGY
   }
+  // compact face records (launch_adjoint_faces_fused): the ten crossing
+  // cotangents mu, then Ac, Bx, By, Bz of the transposed collision
   const unsigned pop_bytes = (unsigned)(P.eN * (long long)sizeof(Real));
-  const RawBuf fx(faces, Q * pop_bytes), fy(faces + Q * P.eN, Q * pop_bytes), fz(faces + 2 * Q * P.eN, Q * pop_bytes);
-  Real pre[2][3][Q];
-  // face cotangents of plane p (ghost-inclusive z slot p + 1) into registers
+  const RawBuf fx(faces, kCompact * pop_bytes), fy(faces + kCompact * P.eN, kCompact * pop_bytes),
+      fz(faces + 2 * kCompact * P.eN, kCompact * pop_bytes);
+  // wall faces among the staged slots: x-face slot 0 / nx, y-face slot 0 / ny
+  bool wlo[2][2], whi[2][2];
+  for (int h = 0; h < 2; h++) {
+    const int pp = t + h * kGT;
+    const int xs = i0 + pp % kGW, ys = j0 + pp / kGW;
+    wlo[h][0] = !P.periodic[0] && xs == 0;
+    whi[h][0] = !P.periodic[0] && xs == nx;
+    wlo[h][1] = !P.periodic[1] && ys == 0;
+    whi[h][1] = !P.periodic[1] && ys == ny;
+  }
+  const Real fw1 = (Real)P.fw1;
+  Real pre[2][3][kCompact];
+  // face records of plane p (ghost-inclusive z slot p + 1) into registers
   auto stage_load = [&](int p) {
     int zs = p + 1;
     zs = zs == 0 ? nz : (zs == nz + 1 ? 1 : zs);
@@ -431,24 +442,37 @@ __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __r
     for (int h = 0; h < 2; h++) {
       const unsigned vb = (zoff + exy[h]) * (unsigned)sizeof(Real);
       const unsigned vx = okx[h] ? vb : kOutOfRange, vy = oky[h] ? vb : kOutOfRange, vz = okp[h] ? vb : kOutOfRange;
-      static_for<Q>([&](auto I) {
-        constexpr int ip = decltype(I)::value;
-        pre[h][0][ip] = fx.ld<Real>(vx, ip * pop_bytes);
-        pre[h][1][ip] = fy.ld<Real>(vy, ip * pop_bytes);
-        pre[h][2][ip] = fz.ld<Real>(vz, ip * pop_bytes);
+      static_for<kCompact>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        pre[h][0][k] = fx.ld<Real>(vx, k * pop_bytes);
+        pre[h][1][k] = fy.ld<Real>(vy, k * pop_bytes);
+        pre[h][2][k] = fz.ld<Real>(vz, k * pop_bytes);
       });
     }
+  };
+  // the 19 reconstructed-value cotangents of each staged face into LDS (a
+  // missing face has an all-zero record and expands to zeros)
+  auto expand_store = [&](auto AI, int h, int pp) {
+    constexpr int A = decltype(AI)::value;
+    Real mu[10], B[3], lamf[Q];
+    static_for<10>([&](auto K) { mu[decltype(K)::value] = pre[h][A][decltype(K)::value]; });
+    B[0] = pre[h][A][11];
+    B[1] = pre[h][A][12];
+    B[2] = pre[h][A][13];
+    const bool lo = A < 2 && wlo[h][A < 2 ? A : 0], hi = A < 2 && whi[h][A < 2 ? A : 0];
+    expand_cotangents<Real, A>(mu, pre[h][A][10], B, fw1, lo, hi, lamf);
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      S[A][ip][pp] = lamf[ip];
+    });
   };
   auto stage_store = [&]() {
     for (int h = 0; h < 2; h++) {
       if (!okp[h]) continue;
       const int pp = t + h * kGT;
-      static_for<Q>([&](auto I) {
-        constexpr int ip = decltype(I)::value;
-        S[0][ip][pp] = pre[h][0][ip];
-        S[1][ip][pp] = pre[h][1][ip];
-        S[2][ip][pp] = pre[h][2][ip];
-      });
+      expand_store(std::integral_constant<int, 0>{}, h, pp);
+      expand_store(std::integral_constant<int, 1>{}, h, pp);
+      expand_store(std::integral_constant<int, 2>{}, h, pp);
     }
   };
 
@@ -645,8 +669,31 @@ hipError_t run_adjoint(Ctx& cx, const void* f, const double* lam_next, double* l
   const KParams& P = cx.kp;
   Real* faces = static_cast<Real*>(cx.adj_faces);
   bool handled = false;
-  hipError_t e = launch_adjoint_faces_fused(cx, f, lam_next, faces, &handled);
+  // FP32 arithmetic with the z-marching gather: the fused adjoint-face pass
+  // writes the compact face records, which the gather expands
+  bool compact = false;
+  if constexpr (!std::is_same<Real, double>::value)
+    compact = gather_z_supported(cx) && adjoint_faces_fused_covers(cx);
+  // Each format leaves some slots unwritten (ghost rows, missing faces) that
+  // must read as 0: clear the arrays when the format changes.
+  const int fmt = compact ? 1 : 0;
+  if (cx.adj_faces_format != fmt) {
+    hipError_t e0 = hipMemsetAsync(cx.adj_faces, 0, (size_t)P.eN * 3 * Q * sizeof(Real), cx.stream);
+    if (e0 != hipSuccess) return e0;
+    cx.adj_faces_format = fmt;
+  }
+  hipError_t e = launch_adjoint_faces_fused(cx, f, lam_next, faces, compact, &handled);
   if (e != hipSuccess) return e;
+  if constexpr (!std::is_same<Real, double>::value) {
+    if (compact) {
+      const int TZ = gather_z_chunk(P);
+      dim3 grid(P.n[0] / kGX, P.n[1] / kGY, (P.n[2] + TZ - 1) / TZ);
+      hipLaunchKernelGGL((k_adj_cells_z<Real, Store>), grid, dim3(kGT), 0, cx.stream, P,
+                         static_cast<const Store*>(f), lam_next, static_cast<const Real*>(cx.adj_faces), lam_prev,
+                         grad, TZ);
+      return hipGetLastError();
+    }
+  }
   if (!handled) {
     if (!cx.fbx) {
       e = cx.alloc(&cx.fbx, (size_t)P.eN * Q * sizeof(Real));
@@ -659,15 +706,6 @@ hipError_t run_adjoint(Ctx& cx, const void* f, const double* lam_next, double* l
     hipLaunchKernelGGL((k_adj_faces<Real, 0>), dim3(nbe), dim3(256), 0, cx.stream, P, fbx, lam_next, faces);
     hipLaunchKernelGGL((k_adj_faces<Real, 1>), dim3(nbe), dim3(256), 0, cx.stream, P, fbx, lam_next, faces);
     hipLaunchKernelGGL((k_adj_faces<Real, 2>), dim3(nbe), dim3(256), 0, cx.stream, P, fbx, lam_next, faces);
-  }
-  if constexpr (!std::is_same<Real, double>::value) {
-    if (gather_z_supported(cx)) {
-      const int TZ = gather_z_chunk(P);
-      dim3 grid(P.n[0] / kGX, P.n[1] / kGY, (P.n[2] + TZ - 1) / TZ);
-      hipLaunchKernelGGL((k_adj_cells_z<Real, Store>), grid, dim3(kGT), 0, cx.stream, P, static_cast<const Store*>(f),
-                         lam_next, static_cast<const Real*>(cx.adj_faces), lam_prev, grad, TZ);
-      return hipGetLastError();
-    }
   }
   for (int b = 0; b < 3; b++) {
     if (!P.periodic[b]) continue;
@@ -703,8 +741,7 @@ hipError_t launch_adjoint(Ctx& c, const void* f, const double* lam_next, double*
     const size_t bytes = (size_t)c.kp.eN * 3 * Q * rb;
     hipError_t e = c.alloc(&c.adj_faces, bytes);
     if (e != hipSuccess) return e;
-    e = hipMemsetAsync(c.adj_faces, 0, bytes, c.stream);
-    if (e != hipSuccess) return e;
+    c.adj_faces_format = -1;  // cleared before first use (run_adjoint)
   }
   switch (c.precision) {
     case 0:

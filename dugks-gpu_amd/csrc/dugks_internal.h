@@ -55,8 +55,15 @@ hipError_t launch_step_twopass(Ctx& c, const void* fin, void* fout);
 hipError_t launch_fbar_ghosts(Ctx& c, const void* fin, void* fbx);
 hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled);
 // Adjoint-face pass of the fused kernel (see adjoint.hip); handled = false
-// when the fused kernel does not cover the configuration.
-hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const double* lam, void* faces, bool* handled);
+// when the fused kernel does not cover the configuration.  compact (FP32
+// arithmetic only): each face stores kCompact values, the ten crossing
+// cotangents mu and the coefficients Ac, Bx, By, Bz of the transposed
+// collision, instead of the 19 reconstructed-value cotangents.
+constexpr int kCompact = 14;
+hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const double* lam, void* faces, bool compact,
+                                      bool* handled);
+// Whether launch_adjoint_faces_fused covers the configuration.
+bool adjoint_faces_fused_covers(const Ctx& c);
 // Planar statistics along axis pa into out (device): [n_pa][9] row sums of
 // u, v, w, uw, uv, vw, uu, vv, ww, then global sums, min and max of
 // rho, u, v, w.  scratch holds stats_scratch_doubles(n_pa) doubles.
@@ -97,7 +104,8 @@ struct Ctx {
   void* fbx = nullptr;        // ghost-inclusive f-bar (two-pass kernels, adjoint)
   double* stats = nullptr;      // planar statistics: partial sums and results
   size_t stats_len = 0;
-  void* adj_faces = nullptr;    // adjoint: 19 face cotangents per face, 3 axes, ghost-inclusive, arithmetic type
+  void* adj_faces = nullptr;    // adjoint face records (19 cotangents, or 14 compact values), 3 axes, ghost-inclusive
+  int adj_faces_format = -1;    // what adj_faces holds: -1 nothing yet, 0 full records, 1 compact records
   double* tab[3] = {nullptr, nullptr, nullptr};
   double* wd[3] = {nullptr, nullptr, nullptr};
   float* tabf[3] = {nullptr, nullptr, nullptr};

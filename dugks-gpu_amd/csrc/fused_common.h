@@ -103,6 +103,34 @@ __device__ constexpr int cross_index(int ip) {
   return -1;
 }
 
+// Cotangents of the 19 reconstructed values of a face normal to axis A from
+// the transposed collision's coefficients: fw1 mu_p + Ac + B.c_p, then the
+// transpose of the wall bounce-back (a population that entered through a
+// wall took its opposite's value, so its cotangent moves to that opposite).
+template <class Real, int A>
+__device__ __forceinline__ void expand_cotangents(const Real (&mu)[10], Real Ac, const Real (&B)[3], Real fw1,
+                                                  bool low_wall, bool high_wall, Real (&lam)[Q]) {
+  Real post[Q];
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    constexpr int k = cross_index<A>(ip);
+    Real v = Ac + B[0] * Real(CX[ip]) + B[1] * Real(CY[ip]) + B[2] * Real(CZ[ip]);
+    if constexpr (k >= 0) v += fw1 * mu[k];
+    post[ip] = v;
+  });
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    Real v = post[ip];
+    if constexpr (cv(A, ip) != 0) {
+      const bool in = (low_wall && cv(A, ip) > 0) || (high_wall && cv(A, ip) < 0);
+      const bool opp_in = (low_wall && cv(A, ip) < 0) || (high_wall && cv(A, ip) > 0);
+      if (in) v = Real(0);
+      if (opp_in) v += post[OPP[ip]];
+    }
+    lam[ip] = v;
+  });
+}
+
 // Reconstructed populations of one face normal to axis A, held as the ten
 // values that cross the face plus the partial moments of the nine that do
 // not (those only enter rho and j of the face collision).
@@ -172,14 +200,11 @@ struct Face {
       out[k] = fw1 * cross[k] + (P0 + Real(3) * fw0 * cF) * feq;
     });
   }
-  // Transpose of collide (after bounce) and bounce: given the cotangent mu
-  // of the ten outputs, the cotangent of the 19 reconstructed values before
-  // bounce-back.  d out_q / d fbar_p = fw1 delta_qp + (d/d rho + c_p.d/d j)
-  // of the equilibrium term, so lam_p = fw1 mu_p + A + B.c_p; a population
-  // that entered through a wall took its opposite's value, so its cotangent
-  // moves to that opposite.
-  __device__ __forceinline__ void adjoint(const Real F[3], Real hdt, Real fw0, Real fw1, Real fw2,
-                                          const Real (&mu)[10], bool low_wall, bool high_wall, Real (&lam)[Q]) {
+  // The transposed collision up to its equilibrium coefficients: the
+  // cotangent of reconstructed value p is fw1 mu_p + Ac + B.c_p (mu_p = 0
+  // for the populations that do not cross), before the bounce transpose.
+  __device__ __forceinline__ void adjoint_coeffs(const Real F[3], Real hdt, Real fw0, Real fw2,
+                                                 const Real (&mu)[10], Real& Ac, Real (&B)[3]) {
     static_for<10>([&](auto K) {
       constexpr int k = decltype(K)::value;
       constexpr int ip = dirs<A>(k);
@@ -201,24 +226,22 @@ struct Face {
     });
     Real g_rho, g_u[3], g_F[3];
     feq_adjoint<Real>(lq, fw2, fw0, rho, u, F, g_rho, g_u, g_F, Crossing<A>{});
-    const Real Ac = g_rho - (g_u[0] * ut[0] + g_u[1] * ut[1] + g_u[2] * ut[2]) * inv;
-    const Real Bx = g_u[0] * inv, By = g_u[1] * inv, Bz = g_u[2] * inv;
-    Real post[Q];
-    static_for<Q>([&](auto I) {
-      constexpr int ip = decltype(I)::value;
-      post[ip] = fw1 * lq[ip] + Ac + Bx * Real(CX[ip]) + By * Real(CY[ip]) + Bz * Real(CZ[ip]);
-    });
-    static_for<Q>([&](auto I) {
-      constexpr int ip = decltype(I)::value;
-      Real v = post[ip];
-      if constexpr (cv(A, ip) != 0) {
-        const bool in = (low_wall && cv(A, ip) > 0) || (high_wall && cv(A, ip) < 0);
-        const bool opp_in = (low_wall && cv(A, ip) < 0) || (high_wall && cv(A, ip) > 0);
-        if (in) v = Real(0);
-        if (opp_in) v += post[OPP[ip]];
-      }
-      lam[ip] = v;
-    });
+    Ac = g_rho - (g_u[0] * ut[0] + g_u[1] * ut[1] + g_u[2] * ut[2]) * inv;
+    B[0] = g_u[0] * inv;
+    B[1] = g_u[1] * inv;
+    B[2] = g_u[2] * inv;
+  }
+  // Transpose of collide (after bounce) and bounce: given the cotangent mu
+  // of the ten outputs, the cotangent of the 19 reconstructed values before
+  // bounce-back.  d out_q / d fbar_p = fw1 delta_qp + (d/d rho + c_p.d/d j)
+  // of the equilibrium term, so lam_p = fw1 mu_p + A + B.c_p; a population
+  // that entered through a wall took its opposite's value, so its cotangent
+  // moves to that opposite.
+  __device__ __forceinline__ void adjoint(const Real F[3], Real hdt, Real fw0, Real fw1, Real fw2,
+                                          const Real (&mu)[10], bool low_wall, bool high_wall, Real (&lam)[Q]) {
+    Real Ac, B[3];
+    adjoint_coeffs(F, hdt, fw0, fw2, mu, Ac, B);
+    expand_cotangents<Real, A>(mu, Ac, B, fw1, low_wall, high_wall, lam);
   }
 };
 

@@ -51,7 +51,7 @@ struct AdjFaceIO {
 // reconstructed and its collision linearised as in the step, and the
 // cotangents of its 19 reconstructed values, driven by lam' (adj.lam), are
 // written to adj.faces; nothing else is written.
-template <class Real, class Store, int TX, int TY, bool ADJ = false>
+template <class Real, class Store, int TX, int TY, bool ADJ = false, bool COMPACT = false>
 __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64) k_fused(
     KParams P, const Store* __restrict__ fin,
                                                           Store* __restrict__ fout, int TZ,
@@ -215,6 +215,20 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
   };
   Real lamc[Q], lamn[Q];
   if constexpr (ADJ) load_lam(z0, lamc);
+  // compact form (COMPACT): per face the ten crossing cotangents mu, then
+  // the coefficients Ac, Bx, By, Bz of the transposed collision
+  auto store_compact = [&](int a, long long slot, const Real (&mu)[10], Real Ac, const Real (&B)[3]) {
+    const RawBuf fb(static_cast<Real*>(adj.faces) + (long long)a * kCompact * P.eN, kCompact * face_pop);
+    const unsigned vb = (unsigned)slot * (unsigned)sizeof(Real);
+    static_for<10>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      fb.st(mu[k], vb, k * face_pop);
+    });
+    fb.st(Ac, vb, 10 * face_pop);
+    fb.st(B[0], vb, 11 * face_pop);
+    fb.st(B[1], vb, 12 * face_pop);
+    fb.st(B[2], vb, 13 * face_pop);
+  };
   auto store_face = [&](int a, long long slot, const Real (&lam)[Q]) {
     const RawBuf fb(static_cast<Real*>(adj.faces) + (long long)a * Q * P.eN, Q * face_pop);
     const unsigned vb = (unsigned)slot * (unsigned)sizeof(Real);
@@ -345,9 +359,16 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
           constexpr int ip = XD[k];
           mu[k] = Real(CX[ip]) * (uR * Lam[ip][ia(r, c + 1)] - uL * Lam[ip][ia(r, c)]);
         });
-        fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, lw, hw, lam);
         const int slot = (gi == 0 && P.periodic[0]) ? nx : gi;
-        store_face(0, ((long long)(p + 1) * e1 + (gj + 1)) * e0 + slot, lam);
+        const long long fs = ((long long)(p + 1) * e1 + (gj + 1)) * e0 + slot;
+        if constexpr (COMPACT) {
+          Real Ac, B[3];
+          fc.adjoint_coeffs(Famb, hdt, fw0, fw2, mu, Ac, B);
+          store_compact(0, fs, mu, Ac, B);
+        } else {
+          fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, lw, hw, lam);
+          store_face(0, fs, lam);
+        }
       } else {
         Real out[10];
         fc.collide(Famb, hdt, fw0, fw1, fw2, out);
@@ -386,9 +407,16 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
           constexpr int ip = YD[k];
           mu[k] = Real(CY[ip]) * (uR * Lam[ip][ia(r, c + 1)] - uL * Lam[ip][ia(r - 1, c + 1)]);
         });
-        fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, lw, hw, lam);
         const int slot = (gj == 0 && P.periodic[1]) ? ny : gj;
-        store_face(1, ((long long)(p + 1) * e1 + slot) * e0 + (gi + 1), lam);
+        const long long fs = ((long long)(p + 1) * e1 + slot) * e0 + (gi + 1);
+        if constexpr (COMPACT) {
+          Real Ac, B[3];
+          fc.adjoint_coeffs(Famb, hdt, fw0, fw2, mu, Ac, B);
+          store_compact(1, fs, mu, Ac, B);
+        } else {
+          fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, lw, hw, lam);
+          store_face(1, fs, lam);
+        }
       } else {
         Real out[10];
         fc.collide(Famb, hdt, fw0, fw1, fw2, out);
@@ -410,8 +438,15 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
           constexpr int ip = ZD[k];
           mu[k] = Real(CZ[ip]) * updz * (lamn[ip] - lamc[ip]);
         });
-        fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, false, false, lam);
-        store_face(2, ((long long)(p + 1) * e1 + (gj + 1)) * e0 + (gi + 1), lam);
+        const long long fs = ((long long)(p + 1) * e1 + (gj + 1)) * e0 + (gi + 1);
+        if constexpr (COMPACT) {
+          Real Ac, B[3];
+          fc.adjoint_coeffs(Famb, hdt, fw0, fw2, mu, Ac, B);
+          store_compact(2, fs, mu, Ac, B);
+        } else {
+          fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, false, false, lam);
+          store_face(2, fs, lam);
+        }
       }
     } else {
       if (interior) zface(zp);
@@ -547,13 +582,13 @@ hipError_t run_fused(Ctx& cx, const void* fin, void* fout) {
   return hipGetLastError();
 }
 
-template <class Real, class Store, int TX, int TY>
+template <class Real, class Store, int TX, int TY, bool COMPACT = false>
 hipError_t run_adj_faces(Ctx& cx, const void* f, const double* lam, void* faces) {
   const KParams& P = cx.kp;
   using T = Tile<TX, TY>;
   const int TZ = z_chunk(cx, TX, TY);
   dim3 grid(P.n[0] / TX, P.n[1] / TY, (P.n[2] + TZ - 1) / TZ);
-  hipLaunchKernelGGL((k_fused<Real, Store, TX, TY, true>), grid, dim3(T::NT), 0, cx.stream, P,
+  hipLaunchKernelGGL((k_fused<Real, Store, TX, TY, true, COMPACT>), grid, dim3(T::NT), 0, cx.stream, P,
                      static_cast<const Store*>(f), static_cast<Store*>(nullptr), TZ, nullptr,
                      AdjFaceIO{lam, faces});
   return hipGetLastError();
@@ -601,13 +636,20 @@ hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled)
   }
 }
 
-hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const double* lam, void* faces, bool* handled) {
+bool adjoint_faces_fused_covers(const Ctx& c) {
   int tx = 32, ty = 4;
-  *handled = false;
-  if (c.kernel == 0 || !fused_supported(c, &tx, &ty)) return hipSuccess;
+  if (c.kernel == 0 || !fused_supported(c, &tx, &ty)) return false;
   // lam' and each face array are addressed with 32-bit buffer offsets
   const double rb = c.precision == 0 ? 8.0 : 4.0;
-  if ((double)Q * c.kp.N * 8.0 >= 2147483648.0 || (double)Q * c.kp.eN * rb >= 2147483648.0) return hipSuccess;
+  return (double)Q * c.kp.N * 8.0 < 2147483648.0 && (double)Q * c.kp.eN * rb < 2147483648.0;
+}
+
+hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const double* lam, void* faces, bool compact,
+                                      bool* handled) {
+  int tx = 32, ty = 4;
+  *handled = false;
+  if (!adjoint_faces_fused_covers(c) || (compact && c.precision == 0)) return hipSuccess;
+  fused_supported(c, &tx, &ty);
   *handled = true;
   const bool tall = tx == 16;
   switch (c.precision) {
@@ -615,9 +657,15 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const double* lam, 
       return tall ? run_adj_faces<double, double, 16, 12>(c, f, lam, faces)
                   : run_adj_faces<double, double, 32, 4>(c, f, lam, faces);
     case 1:
+      if (compact)
+        return tall ? run_adj_faces<float, float, 16, 12, true>(c, f, lam, faces)
+                    : run_adj_faces<float, float, 32, 4, true>(c, f, lam, faces);
       return tall ? run_adj_faces<float, float, 16, 12>(c, f, lam, faces)
                   : run_adj_faces<float, float, 32, 4>(c, f, lam, faces);
     default:
+      if (compact)
+        return tall ? run_adj_faces<float, __half, 16, 12, true>(c, f, lam, faces)
+                    : run_adj_faces<float, __half, 32, 4, true>(c, f, lam, faces);
       return tall ? run_adj_faces<float, __half, 16, 12>(c, f, lam, faces)
                   : run_adj_faces<float, __half, 32, 4>(c, f, lam, faces);
   }
