@@ -130,19 +130,26 @@ def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32", path="split"):
     eng.step(T, istep0=3)
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t
-    # one adjoint step (recompute f-bar+ and ghosts, face cotangents, gather)
-    lam = torch.randn((19, nz, ny, nx), dtype=torch.float64, device="cuda")
-    lam2 = torch.empty_like(lam)
+    # one adjoint step (face cotangents, gather), with lam in FP64 and, for
+    # the FP32 modes, in FP32 (what the sweep uses)
     g = torch.zeros((3, nz, ny, nx), dtype=torch.float64, device="cuda")
-    eng.adjoint_step(lam, lam2, g)
-    torch.cuda.synchronize()
-    nadj = 10
-    t = time.perf_counter()
-    for _ in range(nadj):
-        eng.adjoint_step(lam, lam2, g)
-    torch.cuda.synchronize()
-    t_adj = (time.perf_counter() - t) / nadj
-    del lam, lam2
+    t_adj = {}
+    for lt in (("float64", "float32") if precision != "fp64" else ("float64",)):
+        lam = torch.randn((19, nz, ny, nx), dtype=getattr(torch, lt), device="cuda")
+        lam2 = torch.empty_like(lam)
+        try:
+            eng.adjoint_step(lam, lam2, g)
+        except dugks_amd.DugksError:
+            continue  # FP32 lam not supported on this configuration
+        torch.cuda.synchronize()
+        nadj = 10
+        t = time.perf_counter()
+        for _ in range(nadj):
+            eng.adjoint_step(lam, lam2, g)
+        torch.cuda.synchronize()
+        t_adj[lt] = (time.perf_counter() - t) / nadj
+        del lam, lam2
+    lam_type = "float32" if "float32" in t_adj else "float64"
     eng.set_f_device(f0)
     del f0
     torch.cuda.synchronize()
@@ -157,13 +164,16 @@ def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32", path="split"):
     fwd_step = t_fwd / T
     res = dict(precision=precision, path=path, T=T, interval=int(math.ceil(math.sqrt(T))), loss="wall shear",
                forward_ms_per_step=round(fwd_step * 1e3, 4),
-               adjoint_step_ms=round(t_adj * 1e3, 4),
-               adjoint_over_forward_step=round(t_adj / fwd_step, 3),
+               adjoint_lambda=lam_type,
+               adjoint_step_ms=round(t_adj[lam_type] * 1e3, 4),
+               adjoint_over_forward_step=round(t_adj[lam_type] / fwd_step, 3),
+               adjoint_step_ms_fp64_lambda=round(t_adj["float64"] * 1e3, 4),
                fwd_adj_sweep_s=round(t_fa, 3),
                sweep_over_forward=round(t_fa / t_fwd, 3),
                sweep_device_bytes=peak, device_free_before=free0, device_total=total,
                finite=bool(ok))
-    log(f"adjoint: step {t_adj*1e3:.2f} ms vs forward {fwd_step*1e3:.3f} ms; "
+    log(f"adjoint: step {t_adj[lam_type]*1e3:.2f} ms ({lam_type} lam; {t_adj['float64']*1e3:.2f} ms with float64) "
+        f"vs forward {fwd_step*1e3:.3f} ms; "
         f"{T}-step sweep {t_fa:.2f} s ({t_fa/t_fwd:.1f}x forward), {peak/1e9:.1f} GB")
     return res
 

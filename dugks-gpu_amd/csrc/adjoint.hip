@@ -352,11 +352,10 @@ constexpr int kGP = (kGY + 2) * kGW;  // staged positions per array and populati
 constexpr int kGT = kGX * kGY;        // lanes: one per cell column
 static_assert(kGP <= 2 * kGT, "two staged positions per lane at most");
 
-template <class Real, class Store>
+template <class Real, class Store, class Lam>
 __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __restrict__ fin,
-                                                     const double* __restrict__ lam, const Real* __restrict__ faces,
-                                                     double* __restrict__ lam_prev, double* __restrict__ grad,
-                                                     int TZ) {
+                                                     const Lam* __restrict__ lam, const Real* __restrict__ faces,
+                                                     Lam* __restrict__ lam_prev, double* __restrict__ grad, int TZ) {
   __shared__ Real S[3][Q][kGP];   // face cotangents of plane p: x-faces, y-faces, z-faces p+1/2
   __shared__ Real WXs[5][kGW];    // per staged column: wR for c = -1, 0, +1, then the shift weights
   __shared__ Real WYs[5][kGY + 2];  // per staged row
@@ -556,8 +555,8 @@ __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __r
   });
   const StateBuf<Store> bin(fin, Q * (unsigned)(P.N * (long long)sizeof(Store)));
   const unsigned Nb = (unsigned)(P.N * (long long)sizeof(Store));
-  const RawBuf lamb(lam, (unsigned)(Q * P.N * 8));
-  const unsigned lam_pop = (unsigned)(P.N * 8);
+  const RawBuf lamb(lam, (unsigned)(Q * P.N * sizeof(Lam)));
+  const unsigned lam_pop = (unsigned)(P.N * sizeof(Lam));
 
   for (int p = z0 - 1; p <= zend; p++) {
     stage_load(p);
@@ -599,7 +598,7 @@ __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __r
       Real ff[Q], lp[Q];
       static_for<Q>([&](auto I) {
         constexpr int ip = decltype(I)::value;
-        lp[ip] = (Real)lamb.ld_f64(cell * 8u, ip * lam_pop);
+        lp[ip] = (Real)lamb.ld<Lam>(cell * (unsigned)sizeof(Lam), ip * lam_pop);
         ff[ip] = ld_state<Real, Store>(bin, cell * (unsigned)sizeof(Store), ip * Nb, P, ip);
       });
       static_for<Q>([&](auto I) {
@@ -629,7 +628,7 @@ __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __r
         constexpr int ip = decltype(I)::value;
         const Real v = -lp[ip] * Real(1.0 / 3.0) + bw1 * accP[ip] + Acoef +
                        (g_u[0] * Real(CX[ip]) + g_u[1] * Real(CY[ip]) + g_u[2] * Real(CZ[ip])) * invr;
-        lam_prev[ip * P.N + cl] = (double)v;
+        lam_prev[ip * P.N + cl] = (Lam)v;
       });
       if (grad) {
         for (int a = 0; a < 3; a++) grad[a * P.N + cl] += (double)(g_F[a] + hdt * g_u[a]);
@@ -664,16 +663,20 @@ int gather_z_chunk(const KParams& P) {
   return tz;
 }
 
-template <class Real, class Store>
-hipError_t run_adjoint(Ctx& cx, const void* f, const double* lam_next, double* lam_prev, double* grad, int wall) {
+// One adjoint step.  Lam = double: every configuration; Lam = float (FP32
+// arithmetic with the z-marching gather only): lam' and lam in FP32.
+template <class Real, class Store, class Lam>
+hipError_t run_adjoint(Ctx& cx, const void* f, const Lam* lam_next, Lam* lam_prev, double* grad, int wall) {
   const KParams& P = cx.kp;
   Real* faces = static_cast<Real*>(cx.adj_faces);
+  constexpr bool kLam32 = std::is_same<Lam, float>::value;
   bool handled = false;
   // FP32 arithmetic with the z-marching gather: the fused adjoint-face pass
   // writes the compact face records, which the gather expands
   bool compact = false;
   if constexpr (!std::is_same<Real, double>::value)
     compact = gather_z_supported(cx) && adjoint_faces_fused_covers(cx);
+  if (kLam32 && !compact) return hipErrorNotSupported;
   // Each format leaves some slots unwritten (ghost rows, missing faces) that
   // must read as 0: clear the arrays when the format changes.
   const int fmt = compact ? 1 : 0;
@@ -682,48 +685,55 @@ hipError_t run_adjoint(Ctx& cx, const void* f, const double* lam_next, double* l
     if (e0 != hipSuccess) return e0;
     cx.adj_faces_format = fmt;
   }
-  hipError_t e = launch_adjoint_faces_fused(cx, f, lam_next, faces, compact, &handled);
+  const int format = !compact ? kFacesFull : (kLam32 ? kFacesCompactLam32 : kFacesCompact);
+  hipError_t e = launch_adjoint_faces_fused(cx, f, lam_next, faces, format, &handled);
   if (e != hipSuccess) return e;
   if constexpr (!std::is_same<Real, double>::value) {
     if (compact) {
       const int TZ = gather_z_chunk(P);
       dim3 grid(P.n[0] / kGX, P.n[1] / kGY, (P.n[2] + TZ - 1) / TZ);
-      hipLaunchKernelGGL((k_adj_cells_z<Real, Store>), grid, dim3(kGT), 0, cx.stream, P,
+      hipLaunchKernelGGL((k_adj_cells_z<Real, Store, Lam>), grid, dim3(kGT), 0, cx.stream, P,
                          static_cast<const Store*>(f), lam_next, static_cast<const Real*>(cx.adj_faces), lam_prev,
                          grad, TZ);
       return hipGetLastError();
     }
   }
-  if (!handled) {
-    if (!cx.fbx) {
-      e = cx.alloc(&cx.fbx, (size_t)P.eN * Q * sizeof(Real));
+  if constexpr (!kLam32) {
+    if (!handled) {
+      if (!cx.fbx) {
+        e = cx.alloc(&cx.fbx, (size_t)P.eN * Q * sizeof(Real));
+        if (e != hipSuccess) return e;
+      }
+      e = launch_fbar_ghosts(cx, f, cx.fbx);
       if (e != hipSuccess) return e;
+      const unsigned nbe = (unsigned)((P.eN + 255) / 256);
+      const Real* fbx = static_cast<const Real*>(cx.fbx);
+      hipLaunchKernelGGL((k_adj_faces<Real, 0>), dim3(nbe), dim3(256), 0, cx.stream, P, fbx, lam_next, faces);
+      hipLaunchKernelGGL((k_adj_faces<Real, 1>), dim3(nbe), dim3(256), 0, cx.stream, P, fbx, lam_next, faces);
+      hipLaunchKernelGGL((k_adj_faces<Real, 2>), dim3(nbe), dim3(256), 0, cx.stream, P, fbx, lam_next, faces);
     }
-    e = launch_fbar_ghosts(cx, f, cx.fbx);
-    if (e != hipSuccess) return e;
-    const unsigned nbe = (unsigned)((P.eN + 255) / 256);
-    const Real* fbx = static_cast<const Real*>(cx.fbx);
-    hipLaunchKernelGGL((k_adj_faces<Real, 0>), dim3(nbe), dim3(256), 0, cx.stream, P, fbx, lam_next, faces);
-    hipLaunchKernelGGL((k_adj_faces<Real, 1>), dim3(nbe), dim3(256), 0, cx.stream, P, fbx, lam_next, faces);
-    hipLaunchKernelGGL((k_adj_faces<Real, 2>), dim3(nbe), dim3(256), 0, cx.stream, P, fbx, lam_next, faces);
+    for (int b = 0; b < 3; b++) {
+      if (!P.periodic[b]) continue;
+      const long long cnt = (long long)P.e[(b + 1) % 3] * P.e[(b + 2) % 3] * 3 * Q;
+      hipLaunchKernelGGL((k_face_ghosts<Real>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, cx.stream, P,
+                         faces, b);
+    }
+    const unsigned nbc = (unsigned)((P.n[0] + kCX - 1) / kCX * ((P.n[1] + kCY - 1) / kCY) *
+                                    ((P.n[2] + kCZ - 1) / kCZ));
+    hipLaunchKernelGGL((k_adj_cells<Real, Store>), dim3(nbc), dim3(256), 0, cx.stream, P,
+                       static_cast<const Store*>(f), lam_next, static_cast<const Real*>(cx.adj_faces), lam_prev, grad,
+                       wall);
+    return hipGetLastError();
   }
-  for (int b = 0; b < 3; b++) {
-    if (!P.periodic[b]) continue;
-    const long long cnt = (long long)P.e[(b + 1) % 3] * P.e[(b + 2) % 3] * 3 * Q;
-    hipLaunchKernelGGL((k_face_ghosts<Real>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, cx.stream, P, faces,
-                       b);
-  }
-  const unsigned nbc = (unsigned)((P.n[0] + kCX - 1) / kCX * ((P.n[1] + kCY - 1) / kCY) *
-                                  ((P.n[2] + kCZ - 1) / kCZ));
-  hipLaunchKernelGGL((k_adj_cells<Real, Store>), dim3(nbc), dim3(256), 0, cx.stream, P,
-                     static_cast<const Store*>(f), lam_next, static_cast<const Real*>(cx.adj_faces), lam_prev, grad,
-                     wall);
-  return hipGetLastError();
+  return hipErrorNotSupported;
 }
 
 }  // namespace
 
-hipError_t launch_adjoint(Ctx& c, const void* f, const double* lam_next, double* lam_prev, double* grad_force) {
+namespace {
+
+template <class Lam>
+hipError_t adjoint_step(Ctx& c, const void* f, const Lam* lam_next, Lam* lam_prev, double* grad_force) {
   const Setup& s = c.s;
   if (s.recon != 0) return hipErrorNotSupported;
   int walls = 0, wall = -1;
@@ -733,9 +743,13 @@ hipError_t launch_adjoint(Ctx& c, const void* f, const double* lam_next, double*
       wall = a;
     }
   if (walls > 1) return hipErrorNotSupported;
-  bool handled = false;
-  hipError_t e1 = launch_adjoint_onepass(c, f, lam_next, lam_prev, grad_force, &handled);
-  if (e1 != hipSuccess || handled) return e1;
+  if constexpr (std::is_same<Lam, double>::value) {
+    bool handled = false;
+    hipError_t e1 = launch_adjoint_onepass(c, f, lam_next, lam_prev, grad_force, &handled);
+    if (e1 != hipSuccess || handled) return e1;
+  } else {
+    if (c.precision == 0) return hipErrorNotSupported;
+  }
   const size_t rb = c.precision == 0 ? 8 : 4;
   if (!c.adj_faces) {
     const size_t bytes = (size_t)c.kp.eN * 3 * Q * rb;
@@ -745,12 +759,30 @@ hipError_t launch_adjoint(Ctx& c, const void* f, const double* lam_next, double*
   }
   switch (c.precision) {
     case 0:
-      return run_adjoint<double, double>(c, f, lam_next, lam_prev, grad_force, wall);
+      if constexpr (std::is_same<Lam, double>::value)
+        return run_adjoint<double, double, Lam>(c, f, lam_next, lam_prev, grad_force, wall);
+      else
+        return hipErrorNotSupported;
     case 1:
-      return run_adjoint<float, float>(c, f, lam_next, lam_prev, grad_force, wall);
+      return run_adjoint<float, float, Lam>(c, f, lam_next, lam_prev, grad_force, wall);
     default:
-      return run_adjoint<float, __half>(c, f, lam_next, lam_prev, grad_force, wall);
+      return run_adjoint<float, __half, Lam>(c, f, lam_next, lam_prev, grad_force, wall);
   }
+}
+
+}  // namespace
+
+hipError_t launch_adjoint(Ctx& c, const void* f, const double* lam_next, double* lam_prev, double* grad_force) {
+  return adjoint_step<double>(c, f, lam_next, lam_prev, grad_force);
+}
+
+hipError_t launch_adjoint_lam32(Ctx& c, const void* f, const float* lam_next, float* lam_prev, double* grad_force) {
+  return adjoint_step<float>(c, f, lam_next, lam_prev, grad_force);
+}
+
+bool adjoint_lam32_supported(const Ctx& c) {
+  if (c.precision == 0 || c.s.recon != 0) return false;
+  return gather_z_supported(c) && adjoint_faces_fused_covers(c);
 }
 
 }  // namespace dugks

@@ -715,6 +715,20 @@ int dugks_adjoint_step(dugks_ctx* ctx, const double* lam_next, double* lam_prev,
   return DUGKS_OK;
 }
 
+int dugks_adjoint_step_f32(dugks_ctx* ctx, const float* lam_next, float* lam_prev, double* grad_force) {
+  CHECK_CTX();
+  if (!lam_next || !lam_prev) return fail(ctx, DUGKS_EARG, "null adjoint buffer");
+  Ctx& c = ctx->c;
+  const size_t bytes = (size_t)Q * c.s.ncell() * sizeof(float);
+  const char *a = reinterpret_cast<const char*>(lam_next), *b = reinterpret_cast<const char*>(lam_prev);
+  if (a < b + bytes && b < a + bytes) return fail(ctx, DUGKS_EARG, "lam_prev overlaps lam_next");
+  if (!adjoint_lam32_supported(c))
+    return fail(ctx, DUGKS_EUNSUPPORTED, "FP32 lam needs FP32 arithmetic, nx % 32 == 0, ny % 8 == 0, "
+                                         "periodic uniform z, the fused step and at most one wall axis");
+  HIPCHK(launch_adjoint_lam32(c, c.f[c.cur], lam_next, lam_prev, grad_force));
+  return DUGKS_OK;
+}
+
 int dugks_set_adjoint_path(dugks_ctx* ctx, int path) {
   CHECK_CTX();
   if (path != DUGKS_ADJOINT_AUTO && path != DUGKS_ADJOINT_SPLIT && path != DUGKS_ADJOINT_ONEPASS)
@@ -739,7 +753,11 @@ int dugks_forward_adjoint(dugks_ctx* ctx, int istep0, int T, int loss_kind, int 
   const size_t sb = c.store_bytes;
   // checkpoint store (state at the start of each segment) + segment states
   std::vector<void*> cps(ncp, nullptr), seg(interval, nullptr);
-  double *lam = nullptr, *lam2 = nullptr;
+  // lam in FP32 where the FP32-arithmetic adjoint supports it: its values are
+  // FP32 results either way, so this halves the lam traffic of the sweep
+  const bool lam32 = c.adjoint_path != DUGKS_ADJOINT_ONEPASS && adjoint_lam32_supported(c);
+  const size_t lb = lam32 ? sizeof(float) : sizeof(double);
+  void *lam = nullptr, *lam2 = nullptr;
   double bytes = 0;
   auto cleanup = [&]() {
     for (void* p : cps) hipFree(p);
@@ -764,9 +782,9 @@ int dugks_forward_adjoint(dugks_ctx* ctx, int istep0, int T, int loss_kind, int 
     if ((rc = chk(hipMalloc(&seg[q], nf * sb), "segment alloc"))) return rc;
     bytes += (double)nf * sb;
   }
-  if ((rc = chk(hipMalloc((void**)&lam, nf * sizeof(double)), "lambda alloc"))) return rc;
-  if ((rc = chk(hipMalloc((void**)&lam2, nf * sizeof(double)), "lambda alloc"))) return rc;
-  bytes += 2.0 * nf * sizeof(double);
+  if ((rc = chk(hipMalloc(&lam, nf * lb), "lambda alloc"))) return rc;
+  if ((rc = chk(hipMalloc(&lam2, nf * lb), "lambda alloc"))) return rc;
+  bytes += 2.0 * nf * lb;
   // dval[0]: sum of the loss over the forward sweep (accumulated on the
   // device, read once); dval[1]: discarded values of the reverse sweep
   double* dval = c.red + reduction_scratch_doubles() + 9;
@@ -790,7 +808,7 @@ int dugks_forward_adjoint(dugks_ctx* ctx, int istep0, int T, int loss_kind, int 
   *J = Jsum / T;
   const void* fT = c.f[c.cur];  // final state; the context keeps it
   // reverse sweep
-  if ((rc = chk(hipMemsetAsync(lam, 0, nf * sizeof(double), c.stream), "memset"))) return rc;
+  if ((rc = chk(hipMemsetAsync(lam, 0, nf * lb, c.stream), "memset"))) return rc;
   if ((rc = chk(hipMemsetAsync(dev_grad_force, 0, 3 * N * sizeof(double), c.stream), "memset"))) return rc;
   const double invT = 1.0 / T;
   // Recompute segments from their checkpoints into seg[], then step back.
@@ -809,14 +827,22 @@ int dugks_forward_adjoint(dugks_ctx* ctx, int istep0, int T, int loss_kind, int 
     for (int n = n1 - 1; n >= n0; n--) {
       // lam currently = dJ/df^{n+1}; add the loss term at step n+1
       const void* fnext = (n + 1 < n1) ? seg[n + 1 - n0] : (n1 == T ? fT : cps[q + 1]);
-      if ((rc = chk(launch_loss(c, fnext, loss_kind, dval + 1, lam, invT), "loss grad"))) return rc;
       // lam2 = (df^{n+1}/df^n)^T lam, about f^n = seg[n - n0]
-      if ((rc = chk(launch_adjoint(c, seg[n - n0], lam, lam2, dev_grad_force), "adjoint"))) return rc;
+      if (lam32) {
+        float *l = static_cast<float*>(lam), *l2 = static_cast<float*>(lam2);
+        if ((rc = chk(launch_loss_lam32(c, fnext, loss_kind, dval + 1, l, invT), "loss grad"))) return rc;
+        if ((rc = chk(launch_adjoint_lam32(c, seg[n - n0], l, l2, dev_grad_force), "adjoint"))) return rc;
+      } else {
+        double *l = static_cast<double*>(lam), *l2 = static_cast<double*>(lam2);
+        if ((rc = chk(launch_loss(c, fnext, loss_kind, dval + 1, l, invT), "loss grad"))) return rc;
+        if ((rc = chk(launch_adjoint(c, seg[n - n0], l, l2, dev_grad_force), "adjoint"))) return rc;
+      }
       std::swap(lam, lam2);
     }
   }
   if (dev_grad_f0)
-    if ((rc = chk(hipMemcpyAsync(dev_grad_f0, lam, nf * sizeof(double), hipMemcpyDeviceToDevice, c.stream),
+    if ((rc = chk(lam32 ? launch_widen(c, static_cast<const float*>(lam), dev_grad_f0, (long long)nf)
+                        : hipMemcpyAsync(dev_grad_f0, lam, nf * sizeof(double), hipMemcpyDeviceToDevice, c.stream),
                   "grad copy")))
       return rc;
   if ((rc = chk(hipStreamSynchronize(c.stream), "sync"))) return rc;

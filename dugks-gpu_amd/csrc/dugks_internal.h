@@ -55,12 +55,14 @@ hipError_t launch_step_twopass(Ctx& c, const void* fin, void* fout);
 hipError_t launch_fbar_ghosts(Ctx& c, const void* fin, void* fbx);
 hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled);
 // Adjoint-face pass of the fused kernel (see adjoint.hip); handled = false
-// when the fused kernel does not cover the configuration.  compact (FP32
-// arithmetic only): each face stores kCompact values, the ten crossing
-// cotangents mu and the coefficients Ac, Bx, By, Bz of the transposed
-// collision, instead of the 19 reconstructed-value cotangents.
+// when the fused kernel does not cover the configuration.  Face record
+// formats: kFacesFull, the 19 reconstructed-value cotangents per face, lam'
+// in FP64; kFacesCompact (FP32 arithmetic only), kCompact values per face,
+// the ten crossing cotangents mu and the coefficients Ac, Bx, By, Bz of the
+// transposed collision; kFacesCompactLam32, the same with lam' in FP32.
 constexpr int kCompact = 14;
-hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const double* lam, void* faces, bool compact,
+constexpr int kFacesFull = 0, kFacesCompact = 1, kFacesCompactLam32 = 2;
+hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, void* faces, int format,
                                       bool* handled);
 // Whether launch_adjoint_faces_fused covers the configuration.
 bool adjoint_faces_fused_covers(const Ctx& c);
@@ -77,6 +79,11 @@ hipError_t launch_flow_control(Ctx& c, const void* f);
 hipError_t launch_perturb_force(Ctx& c, int step);
 hipError_t launch_adjoint(Ctx& c, const void* f, const double* lam_next, double* lam_prev,
                           double* grad_force);
+// The same with lam' and lam in FP32; only FP32 arithmetic on the z-marching
+// gather's configurations (adjoint_lam32_supported), else hipErrorNotSupported.
+hipError_t launch_adjoint_lam32(Ctx& c, const void* f, const float* lam_next, float* lam_prev,
+                                double* grad_force);
+bool adjoint_lam32_supported(const Ctx& c);
 // One-pass adjoint step (adjoint_fused.hip); handled = false when it does not
 // cover the configuration.
 hipError_t launch_adjoint_onepass(Ctx& c, const void* f, const double* lam_next, double* lam_prev,
@@ -86,6 +93,11 @@ hipError_t launch_adjoint_onepass(Ctx& c, const void* f, const double* lam_next,
 // to it times grad_acc.
 hipError_t launch_loss(Ctx& c, const void* f, int kind, double* dev_value, double* dev_grad, double grad_acc = 0,
                        bool value_acc = false);
+// The same with the gradient in FP32 (FP32 arithmetic only).
+hipError_t launch_loss_lam32(Ctx& c, const void* f, int kind, double* dev_value, float* dev_grad,
+                             double grad_acc = 0, bool value_acc = false);
+// dst[i] = src[i] (FP32 to FP64), n values.
+hipError_t launch_widen(Ctx& c, const float* src, double* dst, long long n);
 hipError_t launch_axpy(Ctx& c, double a, const double* x, double* y, long long n);
 
 // Library context.

@@ -191,10 +191,10 @@ __global__ void __launch_bounds__(256) k_perturb(KParams P, int step, double xwi
 // The wall-shear loss only involves the two wall planes: unless the gradient
 // array is being initialised (grad set, gacc == 0), the kernel visits those
 // planes only (2 nx nz of the N cells on a y-wall channel).
-template <class Store>
+template <class Store, class G>
 __global__ void __launch_bounds__(kRedThreads) k_loss(KParams P, const Store* __restrict__ f, int kind,
                                                       int aw, int as, double visc, double vol,
-                                                      double nplane, double* grad, double gacc,
+                                                      double nplane, G* grad, double gacc,
                                                       double* partial) {
   double acc[1] = {0};
   double F[3];
@@ -238,9 +238,9 @@ __global__ void __launch_bounds__(kRedThreads) k_loss(KParams P, const Store* __
         val += g * x[ip];
         if (grad) {
           if (gacc == 0.0)
-            grad[ip * P.N + c] = g;
+            grad[ip * P.N + c] = (G)g;
           else if (wall)
-            grad[ip * P.N + c] += gacc * g;
+            grad[ip * P.N + c] = (G)((double)grad[ip * P.N + c] + gacc * g);
         }
       });
       acc[0] += val;
@@ -267,9 +267,9 @@ __global__ void __launch_bounds__(kRedThreads) k_loss(KParams P, const Store* __
                             (CZ[ip] + P.hdt * F[2]) * u[2];
           const double g = (dV / vol) * (cu - 0.5 * uu);
           if (gacc == 0.0)
-            grad[ip * P.N + c] = g;
+            grad[ip * P.N + c] = (G)g;
           else
-            grad[ip * P.N + c] += gacc * g;
+            grad[ip * P.N + c] = (G)((double)grad[ip * P.N + c] + gacc * g);
         });
       }
     }
@@ -280,6 +280,11 @@ __global__ void __launch_bounds__(kRedThreads) k_loss(KParams P, const Store* __
 __global__ void __launch_bounds__(256) k_axpy(double a, const double* __restrict__ x, double* y, long long n) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) y[t] += a * x[t];
+}
+
+__global__ void __launch_bounds__(256) k_widen(const float* __restrict__ x, double* __restrict__ y, long long n) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] = (double)x[t];
 }
 
 // Planar statistics along axis pa (statistics_serial, src/output.cu:223-334),
@@ -423,8 +428,8 @@ hipError_t sums_t(Ctx& c, const void* f, double* out) {
   return hipGetLastError();
 }
 
-template <class Store>
-hipError_t loss_t(Ctx& c, const void* f, int kind, double* value, double* grad, double gacc, bool vacc) {
+template <class Store, class G>
+hipError_t loss_t(Ctx& c, const void* f, int kind, double* value, G* grad, double gacc, bool vacc) {
   const Setup& s = c.s;
   const int aw = s.wall_axis();
   int as = s.stream_axis();
@@ -432,7 +437,7 @@ hipError_t loss_t(Ctx& c, const void* f, int kind, double* value, double* grad, 
   if (kind == 0 && as == aw) as = (aw + 1) % 3;
   double vol = s.width[0] * s.width[1] * s.width[2];
   double nplane = aw >= 0 ? (double)s.ncell() / s.n[aw] : 1.0;
-  hipLaunchKernelGGL((k_loss<Store>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.stream, c.kp,
+  hipLaunchKernelGGL((k_loss<Store, G>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.stream, c.kp,
                      static_cast<const Store*>(f), kind, aw < 0 ? 0 : aw, as, s.visc, vol, nplane, grad,
                      gacc, c.red);
   hipLaunchKernelGGL((k_reduce_final<1>), dim3(1), dim3(kRedThreads), 0, c.stream, c.red, kRedBlocks,
@@ -493,13 +498,25 @@ hipError_t launch_perturb_force(Ctx& c, int step) {
 
 hipError_t launch_loss(Ctx& c, const void* f, int kind, double* dev_value, double* dev_grad, double grad_acc,
                        bool value_acc) {
-  if (c.precision == 0) return loss_t<double>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
-  if (c.precision == 1) return loss_t<float>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
-  return loss_t<__half>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
+  if (c.precision == 0) return loss_t<double, double>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
+  if (c.precision == 1) return loss_t<float, double>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
+  return loss_t<__half, double>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
+}
+
+hipError_t launch_loss_lam32(Ctx& c, const void* f, int kind, double* dev_value, float* dev_grad, double grad_acc,
+                             bool value_acc) {
+  if (c.precision == 0) return hipErrorNotSupported;
+  if (c.precision == 1) return loss_t<float, float>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
+  return loss_t<__half, float>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
 }
 
 hipError_t launch_axpy(Ctx& c, double a, const double* x, double* y, long long n) {
   hipLaunchKernelGGL(k_axpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, a, x, y, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_widen(Ctx& c, const float* src, double* dst, long long n) {
+  hipLaunchKernelGGL(k_widen, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, src, dst, n);
   return hipGetLastError();
 }
 

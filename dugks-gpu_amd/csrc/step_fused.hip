@@ -42,7 +42,7 @@ struct Tile {
 
 // Inputs and outputs of the adjoint-face mode of the fused kernel.
 struct AdjFaceIO {
-  const double* lam;  // lam' = dJ/df(n+1), SoA [19][nz][ny][nx]
+  const void* lam;    // lam' = dJ/df(n+1), SoA [19][nz][ny][nx], FP64 or (LamT = float) FP32
   void* faces;        // out: [3][19][ghost-inclusive face slot], arithmetic type
 };
 
@@ -51,7 +51,7 @@ struct AdjFaceIO {
 // reconstructed and its collision linearised as in the step, and the
 // cotangents of its 19 reconstructed values, driven by lam' (adj.lam), are
 // written to adj.faces; nothing else is written.
-template <class Real, class Store, int TX, int TY, bool ADJ = false, bool COMPACT = false>
+template <class Real, class Store, int TX, int TY, bool ADJ = false, bool COMPACT = false, class LamT = double>
 __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64) k_fused(
     KParams P, const Store* __restrict__ fin,
                                                           Store* __restrict__ fout, int TZ,
@@ -201,16 +201,17 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
   // adjoint-face mode: lam' of population ip at an interior cell (0 for a
   // ghost, cell < 0) and the store of a face's cotangents
   // (both arrays are < 2 GiB, checked at launch: 32-bit buffer offsets)
-  const RawBuf lamb(adj.lam, (unsigned)(Q * P.N * 8));
-  const unsigned lam_pop = (unsigned)(P.N * 8);                 // bytes per population
+  constexpr unsigned kLamBytes = sizeof(LamT);
+  const RawBuf lamb(adj.lam, (unsigned)(Q * P.N * kLamBytes));
+  const unsigned lam_pop = (unsigned)(P.N * kLamBytes);         // bytes per population
   const unsigned face_pop = (unsigned)(P.eN * sizeof(Real));    // bytes per population
   // lam' of the thread's own column at plane k (0 for a wall ghost column)
   const bool own_cell = active && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
   auto load_lam = [&](int k, Real (&l)[Q]) {
-    const unsigned vb = own_cell ? (unsigned)(((long long)k * ny + gj) * nx + gi) * 8u : kOutOfRange;
+    const unsigned vb = own_cell ? (unsigned)(((long long)k * ny + gj) * nx + gi) * kLamBytes : kOutOfRange;
     static_for<Q>([&](auto I) {
       constexpr int ip = decltype(I)::value;
-      if constexpr (ip != 0) l[ip] = (Real)lamb.ld_f64(vb, ip * lam_pop);
+      if constexpr (ip != 0) l[ip] = (Real)lamb.ld<LamT>(vb, ip * lam_pop);
     });
   };
   Real lamc[Q], lamn[Q];
@@ -582,13 +583,13 @@ hipError_t run_fused(Ctx& cx, const void* fin, void* fout) {
   return hipGetLastError();
 }
 
-template <class Real, class Store, int TX, int TY, bool COMPACT = false>
-hipError_t run_adj_faces(Ctx& cx, const void* f, const double* lam, void* faces) {
+template <class Real, class Store, int TX, int TY, bool COMPACT = false, class LamT = double>
+hipError_t run_adj_faces(Ctx& cx, const void* f, const void* lam, void* faces) {
   const KParams& P = cx.kp;
   using T = Tile<TX, TY>;
   const int TZ = z_chunk(cx, TX, TY);
   dim3 grid(P.n[0] / TX, P.n[1] / TY, (P.n[2] + TZ - 1) / TZ);
-  hipLaunchKernelGGL((k_fused<Real, Store, TX, TY, true, COMPACT>), grid, dim3(T::NT), 0, cx.stream, P,
+  hipLaunchKernelGGL((k_fused<Real, Store, TX, TY, true, COMPACT, LamT>), grid, dim3(T::NT), 0, cx.stream, P,
                      static_cast<const Store*>(f), static_cast<Store*>(nullptr), TZ, nullptr,
                      AdjFaceIO{lam, faces});
   return hipGetLastError();
@@ -644,11 +645,11 @@ bool adjoint_faces_fused_covers(const Ctx& c) {
   return (double)Q * c.kp.N * 8.0 < 2147483648.0 && (double)Q * c.kp.eN * rb < 2147483648.0;
 }
 
-hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const double* lam, void* faces, bool compact,
+hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, void* faces, int format,
                                       bool* handled) {
   int tx = 32, ty = 4;
   *handled = false;
-  if (!adjoint_faces_fused_covers(c) || (compact && c.precision == 0)) return hipSuccess;
+  if (!adjoint_faces_fused_covers(c) || (format != kFacesFull && c.precision == 0)) return hipSuccess;
   fused_supported(c, &tx, &ty);
   *handled = true;
   const bool tall = tx == 16;
@@ -657,13 +658,19 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const double* lam, 
       return tall ? run_adj_faces<double, double, 16, 12>(c, f, lam, faces)
                   : run_adj_faces<double, double, 32, 4>(c, f, lam, faces);
     case 1:
-      if (compact)
+      if (format == kFacesCompactLam32)
+        return tall ? run_adj_faces<float, float, 16, 12, true, float>(c, f, lam, faces)
+                    : run_adj_faces<float, float, 32, 4, true, float>(c, f, lam, faces);
+      if (format == kFacesCompact)
         return tall ? run_adj_faces<float, float, 16, 12, true>(c, f, lam, faces)
                     : run_adj_faces<float, float, 32, 4, true>(c, f, lam, faces);
       return tall ? run_adj_faces<float, float, 16, 12>(c, f, lam, faces)
                   : run_adj_faces<float, float, 32, 4>(c, f, lam, faces);
     default:
-      if (compact)
+      if (format == kFacesCompactLam32)
+        return tall ? run_adj_faces<float, __half, 16, 12, true, float>(c, f, lam, faces)
+                    : run_adj_faces<float, __half, 32, 4, true, float>(c, f, lam, faces);
+      if (format == kFacesCompact)
         return tall ? run_adj_faces<float, __half, 16, 12, true>(c, f, lam, faces)
                     : run_adj_faces<float, __half, 32, 4, true>(c, f, lam, faces);
       return tall ? run_adj_faces<float, __half, 16, 12>(c, f, lam, faces)

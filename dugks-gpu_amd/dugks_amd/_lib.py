@@ -70,6 +70,7 @@ PROTOTYPES = {
     "dugks_global_sums": (_i, [_ctxp, _dp]),
     "dugks_adjoint_step": (_i, [_ctxp, _vp, _vp, _vp]),
     "dugks_set_adjoint_path": (_i, [_ctxp, _i]),
+    "dugks_adjoint_step_f32": (_i, [_ctxp, _vp, _vp, _vp]),
     "dugks_loss": (_i, [_ctxp, _i, _dp, _vp]),
     "dugks_forward_adjoint": (_i, [_ctxp, _i, _i, _i, _i, _dp, _vp, _vp, _dp]),
     # host-only helpers (no device needed)

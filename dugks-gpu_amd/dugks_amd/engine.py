@@ -21,8 +21,9 @@ def _ptr(a):
     return a.ctypes.data_as(_dp)
 
 
-def _dev_ptr(t):
-    """Device pointer of a torch tensor (FP64, contiguous) or a raw int."""
+def _dev_ptr(t, dtype="float64"):
+    """Device pointer of a contiguous torch tensor of the given dtype (FP64
+    unless stated), or a raw int."""
     if isinstance(t, int):
         return ctypes.c_void_p(t)
     if t is None:
@@ -30,8 +31,8 @@ def _dev_ptr(t):
     if not t.is_contiguous():
         raise ValueError("device buffer must be contiguous")
     import torch
-    if t.dtype != torch.float64:
-        raise ValueError("device buffer must be float64")
+    if t.dtype != getattr(torch, dtype):
+        raise ValueError(f"device buffer must be {dtype}")
     return ctypes.c_void_p(t.data_ptr())
 
 
@@ -211,8 +212,13 @@ class Engine:
         self._check(self._lib.dugks_set_adjoint_path(self._h, int(path)))
 
     def adjoint_step(self, lam_next, lam_prev, grad_force=None):
-        self._check(self._lib.dugks_adjoint_step(self._h, _dev_ptr(lam_next), _dev_ptr(lam_prev),
-                                                 _dev_ptr(grad_force)))
+        """lam_prev = (df^{n+1}/df^n)^T lam_next about the current state (and
+        grad_force += dJ/dF_cell).  lam_* are device tensors [19, nz, ny, nx]
+        in float64, or float32 (dugks_adjoint_step_f32: FP32 and FP16-store
+        modes on the z-marching gather's configurations)."""
+        lt = "float32" if str(getattr(lam_next, "dtype", "")).endswith("float32") else "float64"
+        fn = self._lib.dugks_adjoint_step_f32 if lt == "float32" else self._lib.dugks_adjoint_step
+        self._check(fn(self._h, _dev_ptr(lam_next, lt), _dev_ptr(lam_prev, lt), _dev_ptr(grad_force)))
 
     def forward_adjoint(self, T, grad_force, loss_kind=_lib.LOSS_WALL_SHEAR, interval=0,
                         istep0=1, grad_f0=None):

@@ -174,6 +174,15 @@ int dugks_global_sums(dugks_ctx* ctx, double out[8]);
 int dugks_adjoint_step(dugks_ctx* ctx, const double* lam_next, double* lam_prev,
                        double* grad_force);
 
+/* dugks_adjoint_step with lam_next and lam_prev in FP32 (device SoA
+ * [19][nz][ny][nx] float; grad_force stays FP64).  For the FP32 and
+ * FP16-store modes on the configurations of the z-marching gather (nx % 32 ==
+ * 0, ny % 8 == 0, periodic uniform z, the fused step, at most one wall axis),
+ * else DUGKS_EUNSUPPORTED.  Those modes compute lam in FP32 arithmetic, so
+ * FP32 storage loses nothing and halves the lam traffic; dugks_forward_adjoint
+ * uses it internally where it applies. */
+int dugks_adjoint_step_f32(dugks_ctx* ctx, const float* lam_next, float* lam_prev, double* grad_force);
+
 /* Adjoint kernel path.  DUGKS_ADJOINT_AUTO (the default) and
  * DUGKS_ADJOINT_SPLIT run the split path (adjoint-face pass, then a per-cell
  * gather), the faster one on MI355X.  DUGKS_ADJOINT_ONEPASS runs the one-pass

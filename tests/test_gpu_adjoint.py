@@ -157,6 +157,74 @@ def test_onepass_adjoint_matches_oracle(rng, case):
     assert np.abs(g - g_ref).max() <= 2e-5 * np.abs(g_ref).max()
 
 
+# configurations of the z-marching gather (nx % 32 == 0, ny % 8 == 0)
+LAM32_CASES = {
+    "channel": dict(nx=64, ny=24, nz=40, periods_y=0, mesh_y="tanh:0.85", dt_rule="min", ambient_fx=2e-4,
+                    xwidth=6.4, ywidth=2.0, zwidth=4.0),
+    "walls_x": dict(nx=64, ny=16, nz=20, periods_x=0, mesh_x="tanh:0.8", dt_rule="min", ambient_fz=2e-4,
+                    xwidth=6.4, ywidth=1.6, zwidth=2.0),
+    "periodic": dict(nx=32, ny=16, nz=8, xwidth=3.2, ywidth=1.6, zwidth=0.8, ambient_fx=1e-4),
+}
+
+
+@pytest.mark.parametrize("case", list(LAM32_CASES))
+@pytest.mark.parametrize("precision", [PREC_FP32, PREC_FP16S], ids=["fp32", "fp16"])
+def test_adjoint_step_f32_lambda_matches_f64_lambda(rng, case, precision):
+    # the FP32 modes compute lam in FP32 arithmetic: with lam' exactly
+    # representable in FP32, FP32 storage of lam' and lam changes no bit
+    eng = _engine(configs.base(visc=0.02, **LAM32_CASES[case]), precision=precision)
+    eng.set_f(random_state(eng.shape, rng))
+    nx, ny, nz = eng.shape
+    eng.set_cell_force(*(1e-4 * rng.standard_normal((3, nz, ny, nx))))
+    dev = torch.device("cuda")
+    lam32 = torch.from_numpy(rng.standard_normal((19, nz, ny, nx)).astype(np.float32)).to(dev)
+    lam64 = lam32.double()
+    out = []
+    for lam in (lam64, lam32):
+        lp = torch.empty_like(lam)
+        g = torch.zeros((3, nz, ny, nx), dtype=torch.float64, device=dev)
+        eng.adjoint_step(lam, lp, g)
+        torch.cuda.synchronize()
+        out.append((lp.double().cpu().numpy(), g.cpu().numpy()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.abs(out[0][0]).max() > 0
+
+
+def test_adjoint_step_f32_lambda_unsupported_configurations():
+    dev = torch.device("cuda")
+    for text, prec in ((configs.base(visc=0.02, **LAM32_CASES["periodic"]), PREC_FP64),
+                       (configs.base(visc=0.02, **ONEPASS_CASES["channel"]), PREC_FP32)):
+        eng = _engine(text, precision=prec)
+        nx, ny, nz = eng.shape
+        lam = torch.zeros((19, nz, ny, nx), dtype=torch.float32, device=dev)
+        with pytest.raises(dugks_amd.DugksError):
+            eng.adjoint_step(lam, torch.empty_like(lam))
+
+
+def test_sweep_with_f32_lambda_matches_f64_lambda(rng):
+    # dugks_forward_adjoint keeps lam in FP32 where it can; the one-pass path
+    # selector falls back to the split path with FP64 lam on this grid (ny %
+    # 12 != 0), which must give the same bits
+    eng, f0, G, T = _adjoint_problem(rng, PREC_FP32, nx=32, ny=16, nz=8, T=12)
+    nx, ny, nz = eng.shape
+    dev = torch.device("cuda")
+    res = []
+    for path in (dugks_amd.ADJOINT_AUTO, dugks_amd.ADJOINT_ONEPASS):
+        eng.set_adjoint_path(path)
+        eng.set_f(f0)
+        eng.set_cell_force(*G)
+        gd = torch.empty((3, nz, ny, nx), dtype=torch.float64, device=dev)
+        g0 = torch.empty((19, nz, ny, nx), dtype=torch.float64, device=dev)
+        J, peak = eng.forward_adjoint(T, gd, interval=4, grad_f0=g0)
+        torch.cuda.synchronize()
+        res.append((J, gd.cpu().numpy(), g0.cpu().numpy(), peak))
+    assert res[0][0] == res[1][0]
+    assert np.array_equal(res[0][1], res[1][1]) and np.abs(res[0][1]).max() > 0
+    assert np.array_equal(res[0][2], res[1][2])
+    assert res[0][3] < res[1][3]  # FP32 lam buffers
+
+
 def test_adjoint_step_rejects_overlapping_buffers(rng):
     eng = _engine(configs.base(visc=0.02, **ONEPASS_CASES["periodic"]), precision=PREC_FP32)
     nx, ny, nz = eng.shape
