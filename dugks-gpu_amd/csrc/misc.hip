@@ -200,7 +200,9 @@ __global__ void __launch_bounds__(kRedThreads) k_loss(KParams P, const Store* __
   double F[3];
   ambient_d(P, F);
   const bool wall_only = kind == 0 && !(grad && gacc == 0.0);
-  const int b1 = (aw + 1) % 3, b2 = (aw + 2) % 3;
+  // in-plane axes of the wall planes, the faster-varying first, so that
+  // consecutive threads read consecutive cells
+  const int b1 = aw == 0 ? 1 : 0, b2 = aw == 2 ? 1 : 2;
   const long long npl = (long long)P.n[b1] * P.n[b2];
   const long long count = wall_only ? (P.n[aw] > 1 ? 2 : 1) * npl : P.N;
   for (long long w = (long long)blockIdx.x * kRedThreads + threadIdx.x; w < count;
