@@ -64,8 +64,10 @@ FUSED_CASES = ("periodic", "walls_y", "walls_x", "tiles", "tall", "tall_walls_x"
 
 @pytest.mark.parametrize("case", list(CASES))
 @pytest.mark.parametrize("kernel", [-1, KERNEL_TWOPASS], ids=["auto", "twopass"])
-@pytest.mark.parametrize("precision,tol", [(PREC_FP64, 1e-12), (PREC_FP32, 2e-5)], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("precision,tol", [(PREC_FP64, 1e-12), (PREC_FP32, 2e-5), (PREC_FP16S, 2e-5)],
+                         ids=["fp64", "fp32", "fp16"])
 def test_adjoint_step_matches_oracle(rng, case, kernel, precision, tol):
+    # FP16 storage: the oracle steps from the decoded stored state
     if kernel == KERNEL_TWOPASS and case not in FUSED_CASES:
         pytest.skip("the automatic choice is the two-pass path here already")
     text = configs.base(visc=0.02, **CASES[case])
