@@ -150,6 +150,10 @@ def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32", path="split"):
         t_adj[lt] = (time.perf_counter() - t) / nadj
         del lam, lam2
     lam_type = "float32" if "float32" in t_adj else "float64"
+    S = {"fp64": 8, "fp32": 4}[precision]
+    lb = 4 if lam_type == "float32" else 8
+    rec = 2 * 3 * 14 * 4 if precision != "fp64" else 2 * 3 * 19 * 8
+    adj_bytes = eng.ncell * (3 * 19 * lb + 2 * 19 * S + rec + 2 * 3 * 8)
     eng.set_f_device(f0)
     del f0
     torch.cuda.synchronize()
@@ -168,6 +172,11 @@ def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32", path="split"):
                adjoint_step_ms=round(t_adj[lam_type] * 1e3, 4),
                adjoint_over_forward_step=round(t_adj[lam_type] / fwd_step, 3),
                adjoint_step_ms_fp64_lambda=round(t_adj["float64"] * 1e3, 4),
+               # algorithmic bytes of one adjoint step (per cell: lam' read twice and lam
+               # written, f read twice, 14-value face records of 3 axes written and read,
+               # the FP64 force gradient read and written)
+               adjoint_alg_bytes_per_step=adj_bytes,
+               adjoint_achieved_gbs=round(adj_bytes / t_adj[lam_type] / 1e9, 1),
                fwd_adj_sweep_s=round(t_fa, 3),
                sweep_over_forward=round(t_fa / t_fwd, 3),
                sweep_device_bytes=peak, device_free_before=free0, device_total=total,
