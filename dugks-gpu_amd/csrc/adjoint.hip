@@ -345,7 +345,9 @@ __global__ void __launch_bounds__(256) k_adj_cells(KParams P, const Store* __res
 // at the in-plane level: a cell beside a wall adds q (or 1 - q) times the
 // in-plane gather at the ghost position, which takes the same z transposes
 // as its own.  Each face value is read from HBM about 1.3 times (the ring)
-// instead of about 2.5 times.
+// instead of about 2.5 times.  The centre-stage adjoint needs only rho and j
+// of the state: with FP32 storage it reads them from the face pass (16 bytes
+// a cell) rather than the 19 values of f.
 constexpr int kGX = 32, kGY = 8;
 constexpr int kGW = kGX + 2;          // staged row length
 constexpr int kGP = (kGY + 2) * kGW;  // staged positions per array and population
@@ -557,6 +559,11 @@ __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __r
   const unsigned Nb = (unsigned)(P.N * (long long)sizeof(Store));
   const RawBuf lamb(lam, (unsigned)(Q * P.N * sizeof(Lam)));
   const unsigned lam_pop = (unsigned)(P.N * sizeof(Lam));
+  // with FP32 storage the face pass has stored rho and j of every cell
+  // behind the records: 16 bytes read per cell instead of 76 of f
+  constexpr bool kMoments = std::is_same<Store, float>::value;
+  const RawBuf mom(faces + 3LL * kCompact * P.eN, (unsigned)(4 * P.N * sizeof(Real)));
+  const unsigned mom_pop = (unsigned)(P.N * sizeof(Real));
 
   for (int p = z0 - 1; p <= zend; p++) {
     stage_load(p);
@@ -595,23 +602,28 @@ __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __r
     if (p - 1 >= z0) {
       const unsigned cell = (unsigned)(p - 1) * plane + colcell;
       const long long cl = (long long)cell;
-      Real ff[Q], lp[Q];
+      Real lp[Q];
       static_for<Q>([&](auto I) {
         constexpr int ip = decltype(I)::value;
         lp[ip] = (Real)lamb.ld<Lam>(cell * (unsigned)sizeof(Lam), ip * lam_pop);
-        ff[ip] = ld_state<Real, Store>(bin, cell * (unsigned)sizeof(Store), ip * Nb, P, ip);
       });
+      Real rho, j[3];
+      if constexpr (kMoments) {
+        rho = mom.ld<Real>(cell * (unsigned)sizeof(Real), 0);
+        j[0] = mom.ld<Real>(cell * (unsigned)sizeof(Real), mom_pop);
+        j[1] = mom.ld<Real>(cell * (unsigned)sizeof(Real), 2 * mom_pop);
+        j[2] = mom.ld<Real>(cell * (unsigned)sizeof(Real), 3 * mom_pop);
+      } else {
+        Real ff[Q];
+        static_for<Q>([&](auto I) {
+          constexpr int ip = decltype(I)::value;
+          ff[ip] = ld_state<Real, Store>(bin, cell * (unsigned)sizeof(Store), ip * Nb, P, ip);
+        });
+        cell_moments<Real>(ff, rho, j);
+      }
       static_for<Q>([&](auto I) {
         constexpr int ip = decltype(I)::value;
         accP[ip] += Real(4.0 / 3.0) * lp[ip];
-      });
-      Real rho = 0, j[3] = {0, 0, 0};
-      static_for<Q>([&](auto I) {
-        constexpr int ip = decltype(I)::value;
-        rho += ff[ip];
-        if constexpr (CX[ip] != 0) j[0] += CX[ip] > 0 ? ff[ip] : -ff[ip];
-        if constexpr (CY[ip] != 0) j[1] += CY[ip] > 0 ? ff[ip] : -ff[ip];
-        if constexpr (CZ[ip] != 0) j[2] += CZ[ip] > 0 ? ff[ip] : -ff[ip];
       });
       Real F[3], u[3], ut[3];
       const Real invr = Real(1) / rho;

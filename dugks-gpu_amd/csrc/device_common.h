@@ -176,22 +176,15 @@ __device__ __forceinline__ float tabv<float>(const KParams& P, int a, int idx) {
 //   rho = sum f, u = j/rho + h F, feq_i(rho, u),
 //   fb_i = bw1 f_i + (bw2 + bw0 (c_i - u).F / RT) feq_i.
 // Written with the +/- pairing of opposite populations, which only reorders
-// the arithmetic.
+// the arithmetic.  This form takes rho and j = sum c f of the cell (from
+// cell_moments below).
 template <class Real>
-__device__ __forceinline__ void centre_fbar(const Real (&f)[Q], const Real F[3], Real hdt, Real bw0,
-                                            Real bw1, Real bw2, Real (&fb)[Q]) {
-  Real rho = 0, jx = 0, jy = 0, jz = 0;
-  static_for<Q>([&](auto I) {
-    constexpr int ip = decltype(I)::value;
-    rho += f[ip];
-    if constexpr (CX[ip] != 0) jx += CX[ip] > 0 ? f[ip] : -f[ip];
-    if constexpr (CY[ip] != 0) jy += CY[ip] > 0 ? f[ip] : -f[ip];
-    if constexpr (CZ[ip] != 0) jz += CZ[ip] > 0 ? f[ip] : -f[ip];
-  });
+__device__ __forceinline__ void centre_fbar(const Real (&f)[Q], Real rho, const Real (&j)[3], const Real F[3],
+                                            Real hdt, Real bw0, Real bw1, Real bw2, Real (&fb)[Q]) {
   const Real inv = rcp_real(rho);
-  const Real ux = jx * inv + hdt * F[0];
-  const Real uy = jy * inv + hdt * F[1];
-  const Real uz = jz * inv + hdt * F[2];
+  const Real ux = j[0] * inv + hdt * F[0];
+  const Real uy = j[1] * inv + hdt * F[1];
+  const Real uz = j[2] * inv + hdt * F[2];
   const Real uu = ux * ux + uy * uy + uz * uz;
   const Real base = Real(1) - Real(1.5) * uu;  // 1 - u.u/(2RT)
   const Real uF = ux * F[0] + uy * F[1] + uz * F[2];
@@ -203,6 +196,28 @@ __device__ __forceinline__ void centre_fbar(const Real (&f)[Q], const Real F[3],
     const Real feq = Real(wgt(ip)) * rho * (base + Real(3) * cu + Real(4.5) * cu * cu);
     fb[ip] = bw1 * f[ip] + (P0 + Real(3) * bw0 * cF) * feq;
   });
+}
+
+// rho = sum_i f_i and j = sum_i c_i f_i of one cell, summed in population order.
+template <class Real>
+__device__ __forceinline__ void cell_moments(const Real (&ff)[Q], Real& rho, Real (&j)[3]) {
+  rho = 0;
+  j[0] = j[1] = j[2] = 0;
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    rho += ff[ip];
+    if constexpr (CX[ip] != 0) j[0] += CX[ip] > 0 ? ff[ip] : -ff[ip];
+    if constexpr (CY[ip] != 0) j[1] += CY[ip] > 0 ? ff[ip] : -ff[ip];
+    if constexpr (CZ[ip] != 0) j[2] += CZ[ip] > 0 ? ff[ip] : -ff[ip];
+  });
+}
+
+template <class Real>
+__device__ __forceinline__ void centre_fbar(const Real (&f)[Q], const Real F[3], Real hdt, Real bw0,
+                                            Real bw1, Real bw2, Real (&fb)[Q]) {
+  Real rho, j[3];
+  cell_moments<Real>(f, rho, j);
+  centre_fbar<Real>(f, rho, j, F, hdt, bw0, bw1, bw2, fb);
 }
 
 // Adjoint of the equilibrium-with-force term (used by the adjoint of the

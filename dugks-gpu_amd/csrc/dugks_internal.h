@@ -60,6 +60,9 @@ hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled)
 // in FP64; kFacesCompact (FP32 arithmetic only), kCompact values per face,
 // the ten crossing cotangents mu and the coefficients Ac, Bx, By, Bz of the
 // transposed collision; kFacesCompactLam32, the same with lam' in FP32.
+// With FP32 storage the compact formats also carry, after the three record
+// arrays (at 3 kCompact eN values), rho, j_x, j_y, j_z of every cell ([4][N]),
+// which the gather uses instead of reading f.
 constexpr int kCompact = 14;
 constexpr int kFacesFull = 0, kFacesCompact = 1, kFacesCompactLam32 = 2;
 hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, void* faces, int format,

@@ -43,7 +43,8 @@ struct Tile {
 // Inputs and outputs of the adjoint-face mode of the fused kernel.
 struct AdjFaceIO {
   const void* lam;    // lam' = dJ/df(n+1), SoA [19][nz][ny][nx], FP64 or (LamT = float) FP32
-  void* faces;        // out: [3][19][ghost-inclusive face slot], arithmetic type
+  void* faces;        // out: [3][19][ghost-inclusive face slot], arithmetic type; COMPACT:
+                      // [3][kCompact][slot], then (FP32 storage) rho, j of every cell [4][N]
 };
 
 // ADJ = false: one DUGKS step fin -> fout.
@@ -273,6 +274,19 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
     });
   };
   if (own) prefetch((z0 + 1) % nz);
+  // compact adjoint-face records with FP32 storage: rho and j of every cell
+  // for the gather, written by the interior lanes from the prefetched f.
+  // A chunk writes its planes z0+1 .. zend (mod nz), so the chunks cover
+  // every plane once.
+  auto store_moments = [&](int k, Real rho, const Real (&jm)[3]) {
+    const RawBuf mb(static_cast<Real*>(adj.faces) + 3LL * kCompact * P.eN, (unsigned)(4 * P.N * sizeof(Real)));
+    const unsigned vb = (((unsigned)k * ny + gj) * nx + gi) * (unsigned)sizeof(Real);
+    const unsigned pop = (unsigned)(P.N * sizeof(Real));
+    mb.st(rho, vb, 0);
+    mb.st(jm[0], vb, pop);
+    mb.st(jm[1], vb, 2 * pop);
+    mb.st(jm[2], vb, 3 * pop);
+  };
   for (int p = z0; p < zend; p++) {
     const int kn = (p + 1) % nz;
     if constexpr (ADJ) load_lam(kn, lamn);
@@ -284,7 +298,12 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
         F[1] += (Real)P.fcell[P.N + cc];
         F[2] += (Real)P.fcell[2 * P.N + cc];
       }
-      centre_fbar<Real>(fpre, F, hdt, (Real)P.bw0, (Real)P.bw1, (Real)P.bw2, fbn);
+      Real rho, jm[3];
+      cell_moments<Real>(fpre, rho, jm);
+      if constexpr (ADJ && COMPACT) {
+        if (interior) store_moments(kn, rho, jm);
+      }
+      centre_fbar<Real>(fpre, rho, jm, F, hdt, (Real)P.bw0, (Real)P.bw1, (Real)P.bw2, fbn);
       if (p + 1 < zend) prefetch((p + 2) % nz);
     } else if (active) {
       fbar_column<Real, Store>(P, bin, Nb, gi, gj, kn, Famb, fbn);
