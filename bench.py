@@ -152,7 +152,7 @@ def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32", path="split"):
     lam_type = "float32" if "float32" in t_adj else "float64"
     S = {"fp64": 8, "fp32": 4}[precision]
     lb = 4 if lam_type == "float32" else 8
-    rec = 2 * 3 * 14 * 4 if precision != "fp64" else 2 * 3 * 19 * 8
+    rec = 2 * 3 * 13 * 4 if precision != "fp64" else 2 * 3 * 19 * 8
     # FP32 storage: the face pass hands rho and j (16 B) to the gather, which
     # then does not read f
     state = 19 * S + 2 * 16 if precision == "fp32" else 2 * 19 * S
@@ -177,7 +177,7 @@ def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32", path="split"):
                adjoint_step_ms_fp64_lambda=round(t_adj["float64"] * 1e3, 4),
                # algorithmic bytes of one adjoint step (per cell: lam' read twice and lam
                # written, f read once and rho, j written and read (FP32; else f read
-               # twice), 14-value face records of 3 axes written and read, the FP64
+               # twice), 13-value face records of 3 axes written and read, the FP64
                # force gradient read and written)
                adjoint_alg_bytes_per_step=adj_bytes,
                adjoint_achieved_gbs=round(adj_bytes / t_adj[lam_type] / 1e9, 1),

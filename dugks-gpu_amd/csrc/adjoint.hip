@@ -418,8 +418,7 @@ __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __r
     okx[h] = okp[h] && Cs <= kGX;  // x-face slots i0 .. i0+kGX
     oky[h] = okp[h] && Rs <= kGY;  // y-face slots j0 .. j0+kGY
   }
-  // compact face records (launch_adjoint_faces_fused): the ten crossing
-  // cotangents mu, then Ac, Bx, By, Bz of the transposed collision
+  // compact face records (launch_adjoint_faces_fused, compact_record)
   const unsigned pop_bytes = (unsigned)(P.eN * (long long)sizeof(Real));
   const RawBuf fx(faces, kCompact * pop_bytes), fy(faces + kCompact * P.eN, kCompact * pop_bytes),
       fz(faces + 2 * kCompact * P.eN, kCompact * pop_bytes);
@@ -455,13 +454,10 @@ __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __r
   // missing face has an all-zero record and expands to zeros)
   auto expand_store = [&](auto AI, int h, int pp) {
     constexpr int A = decltype(AI)::value;
-    Real mu[10], B[3], lamf[Q];
-    static_for<10>([&](auto K) { mu[decltype(K)::value] = pre[h][A][decltype(K)::value]; });
-    B[0] = pre[h][A][11];
-    B[1] = pre[h][A][12];
-    B[2] = pre[h][A][13];
+    Real rec[kCompact], lamf[Q];
+    static_for<kCompact>([&](auto K) { rec[decltype(K)::value] = pre[h][A][decltype(K)::value]; });
     const bool lo = A < 2 && wlo[h][A < 2 ? A : 0], hi = A < 2 && whi[h][A < 2 ? A : 0];
-    expand_cotangents<Real, A>(mu, pre[h][A][10], B, fw1, lo, hi, lamf);
+    expand_record<Real, A>(rec, lo, hi, lamf);
     static_for<Q>([&](auto I) {
       constexpr int ip = decltype(I)::value;
       S[A][ip][pp] = lamf[ip];

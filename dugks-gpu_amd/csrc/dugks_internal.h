@@ -57,13 +57,14 @@ hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled)
 // Adjoint-face pass of the fused kernel (see adjoint.hip); handled = false
 // when the fused kernel does not cover the configuration.  Face record
 // formats: kFacesFull, the 19 reconstructed-value cotangents per face, lam'
-// in FP64; kFacesCompact (FP32 arithmetic only), kCompact values per face,
-// the ten crossing cotangents mu and the coefficients Ac, Bx, By, Bz of the
-// transposed collision; kFacesCompactLam32, the same with lam' in FP32.
+// in FP64; kFacesCompact (FP32 arithmetic only), kCompact values per face:
+// the reconstructed-value cotangents of the ten populations crossing the face
+// (fw1 mu + Ac + B.c), then Ac and the two tangential components of B, from
+// which the nine others follow; kFacesCompactLam32, the same with lam' in FP32.
 // With FP32 storage the compact formats also carry, after the three record
 // arrays (at 3 kCompact eN values), rho, j_x, j_y, j_z of every cell ([4][N]),
 // which the gather uses instead of reading f.
-constexpr int kCompact = 14;
+constexpr int kCompact = 13;
 constexpr int kFacesFull = 0, kFacesCompact = 1, kFacesCompactLam32 = 2;
 hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, void* faces, int format,
                                       bool* handled);
@@ -119,7 +120,7 @@ struct Ctx {
   void* fbx = nullptr;        // ghost-inclusive f-bar (two-pass kernels, adjoint)
   double* stats = nullptr;      // planar statistics: partial sums and results
   size_t stats_len = 0;
-  void* adj_faces = nullptr;    // adjoint face records (19 cotangents, or 14 compact values), 3 axes, ghost-inclusive
+  void* adj_faces = nullptr;    // adjoint face records (19 cotangents, or kCompact values), 3 axes, ghost-inclusive
   int adj_faces_format = -1;    // what adj_faces holds: -1 nothing yet, 0 full records, 1 compact records
   double* tab[3] = {nullptr, nullptr, nullptr};
   double* wd[3] = {nullptr, nullptr, nullptr};

@@ -103,21 +103,11 @@ __device__ constexpr int cross_index(int ip) {
   return -1;
 }
 
-// Cotangents of the 19 reconstructed values of a face normal to axis A from
-// the transposed collision's coefficients: fw1 mu_p + Ac + B.c_p, then the
-// transpose of the wall bounce-back (a population that entered through a
-// wall took its opposite's value, so its cotangent moves to that opposite).
+// Transpose of the wall bounce-back: a population that entered through a
+// wall took its opposite's value, so its cotangent moves to that opposite.
 template <class Real, int A>
-__device__ __forceinline__ void expand_cotangents(const Real (&mu)[10], Real Ac, const Real (&B)[3], Real fw1,
-                                                  bool low_wall, bool high_wall, Real (&lam)[Q]) {
-  Real post[Q];
-  static_for<Q>([&](auto I) {
-    constexpr int ip = decltype(I)::value;
-    constexpr int k = cross_index<A>(ip);
-    Real v = Ac + B[0] * Real(CX[ip]) + B[1] * Real(CY[ip]) + B[2] * Real(CZ[ip]);
-    if constexpr (k >= 0) v += fw1 * mu[k];
-    post[ip] = v;
-  });
+__device__ __forceinline__ void bounce_transpose(const Real (&post)[Q], bool low_wall, bool high_wall,
+                                                 Real (&lam)[Q]) {
   static_for<Q>([&](auto I) {
     constexpr int ip = decltype(I)::value;
     Real v = post[ip];
@@ -129,6 +119,68 @@ __device__ __forceinline__ void expand_cotangents(const Real (&mu)[10], Real Ac,
     }
     lam[ip] = v;
   });
+}
+
+// Cotangents of the 19 reconstructed values of a face normal to axis A from
+// the transposed collision's coefficients: fw1 mu_p + Ac + B.c_p, then the
+// transposed bounce-back.
+template <class Real, int A>
+__device__ __forceinline__ void expand_cotangents(const Real (&mu)[10], Real Ac, const Real (&B)[3], Real fw1,
+                                                  bool low_wall, bool high_wall, Real (&lam)[Q]) {
+  Real post[Q];
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    constexpr int k = cross_index<A>(ip);
+    Real v = Ac + B[0] * Real(CX[ip]) + B[1] * Real(CY[ip]) + B[2] * Real(CZ[ip]);
+    if constexpr (k >= 0) v += fw1 * mu[k];
+    post[ip] = v;
+  });
+  bounce_transpose<Real, A>(post, low_wall, high_wall, lam);
+}
+
+// Compact face record (kCompact values): the cotangents fw1 mu_k + Ac + B.c_p
+// of the ten crossing populations, then Ac and the two components of B
+// tangential to the face.  The nine other populations have c_A = 0, so B_A
+// only enters the crossing ones.
+template <int A>
+__device__ constexpr int tangential(int t) {
+  return t == 0 ? (A == 0 ? 1 : 0) : (A == 2 ? 1 : 2);
+}
+template <class Real, int A>
+__device__ __forceinline__ void compact_record(const Real (&mu)[10], Real Ac, const Real (&B)[3], Real fw1,
+                                               Real (&rec)[kCompact]) {
+  static_for<10>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+    constexpr int ip = dirs<A>(k);
+    Real v = Ac + B[0] * Real(CX[ip]) + B[1] * Real(CY[ip]) + B[2] * Real(CZ[ip]);
+    v += fw1 * mu[k];
+    rec[k] = v;
+  });
+  rec[10] = Ac;
+  rec[11] = B[tangential<A>(0)];
+  rec[12] = B[tangential<A>(1)];
+}
+
+// The 19 reconstructed-value cotangents of a compact record (as
+// expand_cotangents of the coefficients it was made from).
+template <class Real, int A>
+__device__ __forceinline__ void expand_record(const Real (&rec)[kCompact], bool low_wall, bool high_wall,
+                                              Real (&lam)[Q]) {
+  Real B[3];
+  B[A] = Real(0);
+  B[tangential<A>(0)] = rec[11];
+  B[tangential<A>(1)] = rec[12];
+  const Real Ac = rec[10];
+  Real post[Q];
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    constexpr int k = cross_index<A>(ip);
+    if constexpr (k >= 0)
+      post[ip] = rec[k];
+    else
+      post[ip] = Ac + B[0] * Real(CX[ip]) + B[1] * Real(CY[ip]) + B[2] * Real(CZ[ip]);
+  });
+  bounce_transpose<Real, A>(post, low_wall, high_wall, lam);
 }
 
 // Reconstructed populations of one face normal to axis A, held as the ten

@@ -217,19 +217,14 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
   };
   Real lamc[Q], lamn[Q];
   if constexpr (ADJ) load_lam(z0, lamc);
-  // compact form (COMPACT): per face the ten crossing cotangents mu, then
-  // the coefficients Ac, Bx, By, Bz of the transposed collision
-  auto store_compact = [&](int a, long long slot, const Real (&mu)[10], Real Ac, const Real (&B)[3]) {
+  // compact form (COMPACT): per face the kCompact values of compact_record
+  auto store_compact = [&](int a, long long slot, const Real (&rec)[kCompact]) {
     const RawBuf fb(static_cast<Real*>(adj.faces) + (long long)a * kCompact * P.eN, kCompact * face_pop);
     const unsigned vb = (unsigned)slot * (unsigned)sizeof(Real);
-    static_for<10>([&](auto K) {
+    static_for<kCompact>([&](auto K) {
       constexpr int k = decltype(K)::value;
-      fb.st(mu[k], vb, k * face_pop);
+      fb.st(rec[k], vb, k * face_pop);
     });
-    fb.st(Ac, vb, 10 * face_pop);
-    fb.st(B[0], vb, 11 * face_pop);
-    fb.st(B[1], vb, 12 * face_pop);
-    fb.st(B[2], vb, 13 * face_pop);
   };
   auto store_face = [&](int a, long long slot, const Real (&lam)[Q]) {
     const RawBuf fb(static_cast<Real*>(adj.faces) + (long long)a * Q * P.eN, Q * face_pop);
@@ -382,9 +377,10 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
         const int slot = (gi == 0 && P.periodic[0]) ? nx : gi;
         const long long fs = ((long long)(p + 1) * e1 + (gj + 1)) * e0 + slot;
         if constexpr (COMPACT) {
-          Real Ac, B[3];
+          Real Ac, B[3], rec[kCompact];
           fc.adjoint_coeffs(Famb, hdt, fw0, fw2, mu, Ac, B);
-          store_compact(0, fs, mu, Ac, B);
+          compact_record<Real, 0>(mu, Ac, B, fw1, rec);
+          store_compact(0, fs, rec);
         } else {
           fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, lw, hw, lam);
           store_face(0, fs, lam);
@@ -430,9 +426,10 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
         const int slot = (gj == 0 && P.periodic[1]) ? ny : gj;
         const long long fs = ((long long)(p + 1) * e1 + slot) * e0 + (gi + 1);
         if constexpr (COMPACT) {
-          Real Ac, B[3];
+          Real Ac, B[3], rec[kCompact];
           fc.adjoint_coeffs(Famb, hdt, fw0, fw2, mu, Ac, B);
-          store_compact(1, fs, mu, Ac, B);
+          compact_record<Real, 1>(mu, Ac, B, fw1, rec);
+          store_compact(1, fs, rec);
         } else {
           fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, lw, hw, lam);
           store_face(1, fs, lam);
@@ -460,9 +457,10 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
         });
         const long long fs = ((long long)(p + 1) * e1 + (gj + 1)) * e0 + (gi + 1);
         if constexpr (COMPACT) {
-          Real Ac, B[3];
+          Real Ac, B[3], rec[kCompact];
           fc.adjoint_coeffs(Famb, hdt, fw0, fw2, mu, Ac, B);
-          store_compact(2, fs, mu, Ac, B);
+          compact_record<Real, 2>(mu, Ac, B, fw1, rec);
+          store_compact(2, fs, rec);
         } else {
           fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, false, false, lam);
           store_face(2, fs, lam);
