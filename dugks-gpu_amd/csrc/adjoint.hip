@@ -561,7 +561,41 @@ __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __r
   const RawBuf mom(faces + 3LL * kCompact * P.eN, (unsigned)(4 * P.N * sizeof(Real)));
   const unsigned mom_pop = (unsigned)(P.N * sizeof(Real));
 
+  // inputs of the centre stage of one cell: lam', rho and j, the cell force,
+  // the gradient so far
+  Real lp[Q], rho = Real(1), j[3] = {0, 0, 0};
+  double fc[3] = {0, 0, 0}, g0[3] = {0, 0, 0};
+  auto load_centre = [&](unsigned cell) {
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      lp[ip] = (Real)lamb.ld<Lam>(cell * (unsigned)sizeof(Lam), ip * lam_pop);
+    });
+    if constexpr (kMoments) {
+      rho = mom.ld<Real>(cell * (unsigned)sizeof(Real), 0);
+      j[0] = mom.ld<Real>(cell * (unsigned)sizeof(Real), mom_pop);
+      j[1] = mom.ld<Real>(cell * (unsigned)sizeof(Real), 2 * mom_pop);
+      j[2] = mom.ld<Real>(cell * (unsigned)sizeof(Real), 3 * mom_pop);
+    } else {
+      Real ff[Q];
+      static_for<Q>([&](auto I) {
+        constexpr int ip = decltype(I)::value;
+        ff[ip] = ld_state<Real, Store>(bin, cell * (unsigned)sizeof(Store), ip * Nb, P, ip);
+      });
+      cell_moments<Real>(ff, rho, j);
+    }
+    if (P.fcell)
+      for (int a = 0; a < 3; a++) fc[a] = P.fcell[a * P.N + (long long)cell];
+    if (grad)
+      for (int a = 0; a < 3; a++) g0[a] = grad[a * P.N + (long long)cell];
+  };
+  // With FP32 lam and moments the registers allow these loads to be issued
+  // before the plane's face records, so the staging and the gather hide them.
+  constexpr bool kEarly = kMoments && std::is_same<Lam, float>::value;
+
   for (int p = z0 - 1; p <= zend; p++) {
+    if constexpr (kEarly) {
+      if (p - 1 >= z0) load_centre((unsigned)(p - 1) * plane + colcell);
+    }
     stage_load(p);
     __syncthreads();  // the previous plane's gather is done with S
     stage_store();
@@ -598,25 +632,7 @@ __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __r
     if (p - 1 >= z0) {
       const unsigned cell = (unsigned)(p - 1) * plane + colcell;
       const long long cl = (long long)cell;
-      Real lp[Q];
-      static_for<Q>([&](auto I) {
-        constexpr int ip = decltype(I)::value;
-        lp[ip] = (Real)lamb.ld<Lam>(cell * (unsigned)sizeof(Lam), ip * lam_pop);
-      });
-      Real rho, j[3];
-      if constexpr (kMoments) {
-        rho = mom.ld<Real>(cell * (unsigned)sizeof(Real), 0);
-        j[0] = mom.ld<Real>(cell * (unsigned)sizeof(Real), mom_pop);
-        j[1] = mom.ld<Real>(cell * (unsigned)sizeof(Real), 2 * mom_pop);
-        j[2] = mom.ld<Real>(cell * (unsigned)sizeof(Real), 3 * mom_pop);
-      } else {
-        Real ff[Q];
-        static_for<Q>([&](auto I) {
-          constexpr int ip = decltype(I)::value;
-          ff[ip] = ld_state<Real, Store>(bin, cell * (unsigned)sizeof(Store), ip * Nb, P, ip);
-        });
-        cell_moments<Real>(ff, rho, j);
-      }
+      if constexpr (!kEarly) load_centre(cell);
       static_for<Q>([&](auto I) {
         constexpr int ip = decltype(I)::value;
         accP[ip] += Real(4.0 / 3.0) * lp[ip];
@@ -625,7 +641,7 @@ __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __r
       const Real invr = Real(1) / rho;
       for (int a = 0; a < 3; a++) {
         F[a] = Famb[a];
-        if (P.fcell) F[a] += (Real)P.fcell[a * P.N + cl];
+        if (P.fcell) F[a] += (Real)fc[a];
         ut[a] = j[a] * invr;
         u[a] = ut[a] + hdt * F[a];
       }
@@ -639,7 +655,7 @@ __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __r
         lam_prev[ip * P.N + cl] = (Lam)v;
       });
       if (grad) {
-        for (int a = 0; a < 3; a++) grad[a * P.N + cl] += (double)(g_F[a] + hdt * g_u[a]);
+        for (int a = 0; a < 3; a++) grad[a * P.N + cl] = g0[a] + (double)(g_F[a] + hdt * g_u[a]);
       }
     }
     static_for<Q>([&](auto I) {

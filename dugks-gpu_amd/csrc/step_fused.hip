@@ -206,10 +206,14 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
   const RawBuf lamb(adj.lam, (unsigned)(Q * P.N * kLamBytes));
   const unsigned lam_pop = (unsigned)(P.N * kLamBytes);         // bytes per population
   const unsigned face_pop = (unsigned)(P.eN * sizeof(Real));    // bytes per population
-  // lam' of the thread's own column at plane k (0 for a wall ghost column)
-  const bool own_cell = active && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+  // lam' of the thread's own column at plane k (0 for a wall ghost column).
+  // The pass computes the -x and -y faces of its cells (and a wall's +x, +y
+  // face, whose outside column is a ghost), so the right halo column, the top
+  // halo row and the bottom-left corner never need lam' and load none.
+  const bool lam_cell = active && gi >= 0 && gi < nx && gj >= 0 && gj < ny && c < TX && r <= TY &&
+                        !(c < 0 && r == 0);
   auto load_lam = [&](int k, Real (&l)[Q]) {
-    const unsigned vb = own_cell ? (unsigned)(((long long)k * ny + gj) * nx + gi) * kLamBytes : kOutOfRange;
+    const unsigned vb = lam_cell ? (unsigned)(((long long)k * ny + gj) * nx + gi) * kLamBytes : kOutOfRange;
     static_for<Q>([&](auto I) {
       constexpr int ip = decltype(I)::value;
       if constexpr (ip != 0) l[ip] = (Real)lamb.ld<LamT>(vb, ip * lam_pop);
