@@ -51,6 +51,9 @@ CASES = {
     "walls_z": dict(nx=8, ny=6, nz=10, periods_z=0, ambient_fy=2e-4, xwidth=0.8, ywidth=0.6, zwidth=1.0),
     # cases the fused adjoint-face pass covers (nx % 32 == 0, z periodic)
     "walls_x": dict(nx=32, ny=8, nz=4, periods_x=0, ambient_fz=2e-4, xwidth=3.2, ywidth=0.8, zwidth=0.4),
+    # x walls across two tiles: the inner tile edges take lam' from the halo column
+    "walls_x_tiles": dict(nx=64, ny=8, nz=6, periods_x=0, mesh_x="tanh:0.8", dt_rule="min", ambient_fz=2e-4,
+                          xwidth=6.4, ywidth=0.8, zwidth=0.6),
     "tiles": dict(nx=64, ny=16, nz=80, periods_y=0, mesh_y="tanh:0.85", dt_rule="min", ambient_fx=2e-4,
                   xwidth=6.4, ywidth=2.0, zwidth=8.0),
     # 16 x 12 tiles
@@ -59,7 +62,7 @@ CASES = {
     "tall_walls_x": dict(nx=16, ny=12, nz=6, periods_x=0, ambient_fz=2e-4, xwidth=1.6, ywidth=1.2, zwidth=0.6),
     "tall_periodic": dict(nx=32, ny=24, nz=6, xwidth=3.2, ywidth=2.4, zwidth=0.6, ambient_fx=1e-4),
 }
-FUSED_CASES = ("periodic", "walls_y", "walls_x", "tiles", "tall", "tall_walls_x", "tall_periodic")
+FUSED_CASES = ("periodic", "walls_y", "walls_x", "walls_x_tiles", "tiles", "tall", "tall_walls_x", "tall_periodic")
 
 
 @pytest.mark.parametrize("case", list(CASES))
