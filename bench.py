@@ -111,7 +111,7 @@ def run_mode(dugks_amd, configs, torch, precision, args, np=None):
     return res, stats
 
 
-def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32", path="split"):
+def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32"):
     """Forward + adjoint over T steps of the same 256x192x256 channel with a
     constant driving force (the adjoint does not differentiate the flow-rate
     controller): wall-shear loss, sqrt(T) checkpoints in HBM, gradient with
@@ -120,7 +120,6 @@ def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32", path="split"):
     prec = {"fp64": dugks_amd.PREC_FP64, "fp32": dugks_amd.PREC_FP32}[precision]
     text = configs.large_channel(precision=precision, flow_rate=0)
     eng = dugks_amd.Engine(text, precision=prec, device=0, stream=torch.cuda.current_stream().cuda_stream)
-    eng.set_adjoint_path({"split": dugks_amd.ADJOINT_SPLIT, "onepass": dugks_amd.ADJOINT_ONEPASS}[path])
     nz, ny, nx = eng.shape[2], eng.shape[1], eng.shape[0]
     f0 = torch.empty((19, nz, ny, nx), dtype=torch.float64, device="cuda")
     eng.get_f_device(f0)
@@ -165,11 +164,12 @@ def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32", path="split"):
     J, peak = eng.forward_adjoint(T, g)
     torch.cuda.synchronize()
     t_fa = time.perf_counter() - t
+    pool = eng.sweep_bytes()
     gmax = float(g.abs().max())
     ok = math.isfinite(J) and math.isfinite(gmax)
     eng.close()
     fwd_step = t_fwd / T
-    res = dict(precision=precision, path=path, T=T, interval=int(math.ceil(math.sqrt(T))), loss="wall shear",
+    res = dict(precision=precision, T=T, interval=int(math.ceil(math.sqrt(T))), loss="wall shear",
                forward_ms_per_step=round(fwd_step * 1e3, 4),
                adjoint_lambda=lam_type,
                adjoint_step_ms=round(t_adj[lam_type] * 1e3, 4),
@@ -183,18 +183,56 @@ def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32", path="split"):
                adjoint_achieved_gbs=round(adj_bytes / t_adj[lam_type] / 1e9, 1),
                fwd_adj_sweep_s=round(t_fa, 3),
                sweep_over_forward=round(t_fa / t_fwd, 3),
-               sweep_device_bytes=peak, device_free_before=free0, device_total=total,
+               # device memory in use at the end of the sweep (hipMemGetInfo: the
+               # engine's state, face records and checkpoint pool, and torch's
+               # cached blocks) and the checkpoint pool alone
+               sweep_device_in_use_bytes=peak, sweep_pool_bytes=pool,
+               device_free_before=free0, device_total=total,
                finite=bool(ok))
     log(f"adjoint: step {t_adj[lam_type]*1e3:.2f} ms ({lam_type} lam; {t_adj['float64']*1e3:.2f} ms with float64) "
         f"vs forward {fwd_step*1e3:.3f} ms; "
-        f"{T}-step sweep {t_fa:.2f} s ({t_fa/t_fwd:.1f}x forward), {peak/1e9:.1f} GB")
+        f"{T}-step sweep {t_fa:.2f} s ({t_fa/t_fwd:.1f}x forward), pool {pool/1e9:.1f} GB, "
+        f"{peak/1e9:.1f} GB in use")
     return res
+
+
+def host_cpu():
+    """CPU model, the physical cores of the CPUs this process may run on, and
+    the host's CPU share for this job (OMP_NUM_THREADS when the launcher sets
+    it): (model, physical_cores, share)."""
+    model = "unknown"
+    core_of = {}
+    cpu, phys, core = None, None, None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh.read().splitlines() + [""]:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    cpu = int(v)
+                elif k == "model name":
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not line.strip() and cpu is not None:
+                    core_of[cpu] = (phys, core)
+                    cpu, phys, core = None, None, None
+    except OSError:
+        pass
+    allowed = os.sched_getaffinity(0)
+    cores = len({core_of.get(c, ("?", c)) for c in allowed}) or len(allowed)
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return model, cores, share
 
 
 def cpu_baseline(configs, threads, target_s=15.0):
     """FP64 oracle (the reference algorithm restated in C++/OpenMP) on a
     z-slab of the same workload: 256 x 192 x 32 cells, same mesh and dt,
-    stepped in batches of 5 until about target_s seconds of CPU work."""
+    stepped in batches of 5 until about target_s seconds of CPU work, one
+    OpenMP thread per physical core (OMP_PROC_BIND=close, set before the
+    OpenMP runtime starts)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import dugks_amd
@@ -216,9 +254,13 @@ def cpu_baseline(configs, threads, target_s=15.0):
         n += 5
     dt = time.perf_counter() - t
     cells = int(np.prod(dims))
+    model, phys, share = host_cpu()
     return dict(value=cells * n / dt / 1e6, unit="MLUPS", cores=threads, kind="port",
+                cpu_model=model, physical_cores=phys, cpu_share=share or None,
+                omp_proc_bind=os.environ.get("OMP_PROC_BIND"),
                 sample=f"FP64 oracle, {n} steps of a 256x192x{nzs} z-slab of the workload "
-                       f"({cells} cells, same stretched mesh and dt), {dt:.1f} s")
+                       f"({cells} cells, same stretched mesh and dt), {dt:.1f} s, "
+                       f"{threads} OpenMP threads on {model}")
 
 
 def study_tgv(dugks_amd, configs, torch, np, steps, every=10):
@@ -363,13 +405,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--modes", default="fp32,fp16,fp64")
     ap.add_argument("--kernel", default="auto", choices=["auto", "fused", "twopass"])
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="OpenMP threads of the CPU baseline (0: one per physical core of this "
+                         "process's CPU set, capped at the job's CPU share OMP_NUM_THREADS)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--adj-steps", type=int, default=1000,
                     help="steps of the forward+adjoint sweep on the large channel (0: skip)")
-    ap.add_argument("--adj-path", default="split", choices=["split", "onepass"],
-                    help="adjoint kernels of the adjoint leg (the split path is the default)")
     ap.add_argument("--study", choices=["tgv", "channel", "adjoint"],
                     help="instead of the benchmark: the FP32 / FP16-store error study on the "
                          "Taylor-Green vortex or the 128^3 channel (prints one JSON line)")
@@ -382,6 +424,10 @@ def main():
         print("bench.py: this build runs on one GPU only (the domain is not sharded); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    # the CPU baseline's OpenMP threads stay on their cores; libgomp reads
+    # this when it starts, which is after this point
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import torch
     if args.copy_calibration:
         x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
@@ -423,12 +469,13 @@ def main():
     stats.clear()
     adjoint = None
     if args.adj_steps > 0:
-        adjoint = adjoint_leg(dugks_amd, configs, torch, args.adj_steps, path=args.adj_path)
+        adjoint = adjoint_leg(dugks_amd, configs, torch, args.adj_steps)
     primary = "fp32" if "fp32" in results else modes[0]
     r = results[primary]
     cpu = None
     if not args.no_cpu:
-        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        _, phys, share = host_cpu()
+        threads = args.cpu_threads or (min(phys, share) if share else phys)
         log(f"cpu baseline on {threads} threads")
         cpu = cpu_baseline(configs, threads)
         log(f"cpu baseline {cpu['value']:.2f} MLUPS")

@@ -21,7 +21,6 @@
 //      stages each plane of face cotangents in LDS and marches along z;
 //      k_adj_cells (every other case) runs one thread per cell over face
 //      arrays with periodic ghost copies (k_face_ghosts).
-// The fused step's tiles also have a one-pass form (adjoint_fused.hip).
 // Deterministic (no atomics).  At most one wall axis.
 #include "device_common.h"
 
@@ -767,11 +766,7 @@ hipError_t adjoint_step(Ctx& c, const void* f, const Lam* lam_next, Lam* lam_pre
       wall = a;
     }
   if (walls > 1) return hipErrorNotSupported;
-  if constexpr (std::is_same<Lam, double>::value) {
-    bool handled = false;
-    hipError_t e1 = launch_adjoint_onepass(c, f, lam_next, lam_prev, grad_force, &handled);
-    if (e1 != hipSuccess || handled) return e1;
-  } else {
+  if constexpr (!std::is_same<Lam, double>::value) {
     if (c.precision == 0) return hipErrorNotSupported;
   }
   const size_t rb = c.precision == 0 ? 8 : 4;

@@ -26,6 +26,8 @@ namespace {
 
 thread_local std::string g_init_err;
 
+void release_sweep(Ctx& c);
+
 int fail(dugks_ctx* ctx, int code, const std::string& msg) {
   if (ctx) ctx->c.err = msg;
   return code;
@@ -378,6 +380,7 @@ void dugks_free(dugks_ctx* ctx) {
   hipFree(c.fcell);
   hipFree(c.red);
   hipFree(c.flow_partial);
+  release_sweep(c);
   for (int q = 0; q < 2; q++)
     if (c.tev[q]) hipEventDestroy(c.tev[q]);
   if (c.own_stream && c.stream) hipStreamDestroy(c.stream);
@@ -447,6 +450,12 @@ int dugks_sync(dugks_ctx* ctx) {
 int dugks_get_dims(const dugks_ctx* ctx, int dims[3]) {
   if (!ctx || !dims) return DUGKS_EARG;
   for (int a = 0; a < 3; a++) dims[a] = ctx->c.s.n[a];
+  return DUGKS_OK;
+}
+
+int dugks_get_device(const dugks_ctx* ctx, int* device) {
+  if (!ctx || !device) return DUGKS_EARG;
+  *device = ctx->c.device;
   return DUGKS_OK;
 }
 
@@ -535,7 +544,10 @@ int dugks_get_f_device(dugks_ctx* ctx, double* dev_f) {
 int dugks_set_f_device(dugks_ctx* ctx, const double* dev_f) {
   CHECK_CTX();
   if (!dev_f) return fail(ctx, DUGKS_EARG, "null device pointer");
-  HIPCHK(launch_import(ctx->c, dev_f, ctx->c.f[ctx->c.cur]));
+  Ctx& c = ctx->c;
+  HIPCHK(launch_import(c, dev_f, c.f[c.cur]));
+  // the flow-rate controller restarts from the new state (stream-ordered)
+  HIPCHK(launch_fill(c, c.famb_dev + 3, 0.0, 2));
   return DUGKS_OK;
 }
 
@@ -729,113 +741,161 @@ int dugks_adjoint_step_f32(dugks_ctx* ctx, const float* lam_next, float* lam_pre
   return DUGKS_OK;
 }
 
-int dugks_set_adjoint_path(dugks_ctx* ctx, int path) {
-  CHECK_CTX();
-  if (path != DUGKS_ADJOINT_AUTO && path != DUGKS_ADJOINT_SPLIT && path != DUGKS_ADJOINT_ONEPASS)
-    return fail(ctx, DUGKS_EARG, "bad adjoint path");
-  ctx->c.adjoint_path = path;
-  return DUGKS_OK;
+namespace {
+
+void release_sweep(Ctx& c) {
+  SweepPool& w = c.sweep;
+  for (void* p : w.cps) hipFree(p);
+  for (void* p : w.seg) hipFree(p);
+  hipFree(w.fT);
+  hipFree(w.lam[0]);
+  hipFree(w.lam[1]);
+  c.bytes_allocated -= w.bytes;
+  w = SweepPool();
 }
 
-int dugks_forward_adjoint(dugks_ctx* ctx, int istep0, int T, int loss_kind, int interval, double* J,
-                          double* dev_grad_force, double* dev_grad_f0, double* peak_bytes) {
+// Grow a list of state buffers to n entries (existing ones are kept).
+hipError_t pool_states(Ctx& c, std::vector<void*>& v, size_t n, size_t bytes) {
+  while (v.size() < n) {
+    void* p = nullptr;
+    hipError_t e = c.alloc(&p, bytes);
+    if (e != hipSuccess) return e;
+    c.sweep.bytes += (double)bytes;
+    v.push_back(p);
+  }
+  return hipSuccess;
+}
+
+// lam in FP32 where the FP32-arithmetic adjoint supports it: its values are
+// FP32 results either way, so this halves the lam traffic of the sweep
+bool sweep_lam32(const Ctx& c) { return adjoint_lam32_supported(c); }
+
+}  // namespace
+
+int dugks_forward_record(dugks_ctx* ctx, int istep0, int T, int loss_kind, int interval, double* J) {
   CHECK_CTX();
-  if (T <= 0 || !J || !dev_grad_force || (loss_kind != 0 && loss_kind != 1))
-    return fail(ctx, DUGKS_EARG, "bad forward_adjoint arguments");
+  if (T <= 0 || !J || (loss_kind != 0 && loss_kind != 1)) return fail(ctx, DUGKS_EARG, "bad forward_record arguments");
   Ctx& c = ctx->c;
   if (loss_kind == 0 && c.s.wall_axis() < 0) return fail(ctx, DUGKS_EUNSUPPORTED, "wall shear needs a wall axis");
   if (c.s.flow_rate != 0 || c.s.perturb)
     return fail(ctx, DUGKS_EUNSUPPORTED, "adjoint with flow-rate control or perturb forcing is not supported");
-  const size_t N = c.s.ncell();
-  const size_t nf = N * Q;
-  if (interval <= 0) interval = (int)std::ceil(std::sqrt((double)T));
-  const int ncp = (T + interval - 1) / interval;  // checkpoints at steps 0, K, 2K, ...
+  SweepPool& w = c.sweep;
+  w.recorded = false;
+  const size_t nf = c.s.ncell() * Q;
   const size_t sb = c.store_bytes;
-  // checkpoint store (state at the start of each segment) + segment states
-  std::vector<void*> cps(ncp, nullptr), seg(interval, nullptr);
-  // lam in FP32 where the FP32-arithmetic adjoint supports it: its values are
-  // FP32 results either way, so this halves the lam traffic of the sweep
-  const bool lam32 = c.adjoint_path != DUGKS_ADJOINT_ONEPASS && adjoint_lam32_supported(c);
-  const size_t lb = lam32 ? sizeof(float) : sizeof(double);
-  void *lam = nullptr, *lam2 = nullptr;
-  double bytes = 0;
-  auto cleanup = [&]() {
-    for (void* p : cps) hipFree(p);
-    for (void* p : seg) hipFree(p);
-    hipFree(lam);
-    hipFree(lam2);
-  };
+  if (interval <= 0) interval = (int)std::ceil(std::sqrt((double)T));
+  interval = std::min(interval, T);
+  const int ncp = (T + interval - 1) / interval;  // checkpoints at steps 0, K, 2K, ...
   auto chk = [&](hipError_t e, const char* what) {
-    if (e != hipSuccess) {
-      cleanup();
+    if (e != hipSuccess)
       return fail(ctx, e == hipErrorOutOfMemory ? DUGKS_ENOMEM : DUGKS_EHIP,
                   std::string(what) + ": " + hipGetErrorString(e));
-    }
     return DUGKS_OK;
   };
   int rc;
-  for (int q = 0; q < ncp; q++) {
-    if ((rc = chk(hipMalloc(&cps[q], nf * sb), "checkpoint alloc"))) return rc;
-    bytes += (double)nf * sb;
+  if ((rc = chk(pool_states(c, w.cps, (size_t)ncp, nf * sb), "checkpoint alloc"))) return rc;
+  if (!w.fT) {
+    if ((rc = chk(c.alloc(&w.fT, nf * sb), "final-state alloc"))) return rc;
+    w.bytes += (double)nf * sb;
   }
-  for (int q = 0; q < interval; q++) {
-    if ((rc = chk(hipMalloc(&seg[q], nf * sb), "segment alloc"))) return rc;
-    bytes += (double)nf * sb;
-  }
-  if ((rc = chk(hipMalloc(&lam, nf * lb), "lambda alloc"))) return rc;
-  if ((rc = chk(hipMalloc(&lam2, nf * lb), "lambda alloc"))) return rc;
-  bytes += 2.0 * nf * lb;
-  // dval[0]: sum of the loss over the forward sweep (accumulated on the
-  // device, read once); dval[1]: discarded values of the reverse sweep
+  // dval[0]: the loss summed over the forward sweep on the device, read once
   double* dval = c.red + reduction_scratch_doubles() + 9;
-  if ((rc = chk(hipMemsetAsync(dval, 0, 2 * sizeof(double), c.stream), "memset"))) return rc;
-  // forward sweep: checkpoints and the loss
+  if ((rc = chk(launch_fill(c, dval, 0.0, 1), "clear"))) return rc;
   for (int n = 0; n < T; n++) {
     if (n % interval == 0)
-      if ((rc = chk(hipMemcpyAsync(cps[n / interval], c.f[c.cur], nf * sb, hipMemcpyDeviceToDevice, c.stream),
+      if ((rc = chk(hipMemcpyAsync(w.cps[n / interval], c.f[c.cur], nf * sb, hipMemcpyDeviceToDevice, c.stream),
                     "checkpoint copy")))
         return rc;
-    if ((rc = step_once(ctx, istep0 + n))) {
-      cleanup();
-      return rc;
-    }
+    if ((rc = step_once(ctx, istep0 + n))) return rc;
     if ((rc = chk(launch_loss(c, c.f[c.cur], loss_kind, dval, nullptr, 0, true), "loss"))) return rc;
   }
+  if ((rc = chk(hipMemcpyAsync(w.fT, c.f[c.cur], nf * sb, hipMemcpyDeviceToDevice, c.stream), "final copy")))
+    return rc;
   double Jsum = 0;
   if ((rc = chk(hipMemcpyAsync(&Jsum, dval, sizeof(double), hipMemcpyDeviceToHost, c.stream), "loss copy")))
     return rc;
   if ((rc = chk(hipStreamSynchronize(c.stream), "sync"))) return rc;
   *J = Jsum / T;
-  const void* fT = c.f[c.cur];  // final state; the context keeps it
-  // reverse sweep
-  if ((rc = chk(hipMemsetAsync(lam, 0, nf * lb, c.stream), "memset"))) return rc;
-  if ((rc = chk(hipMemsetAsync(dev_grad_force, 0, 3 * N * sizeof(double), c.stream), "memset"))) return rc;
+  w.istep0 = istep0;
+  w.T = T;
+  w.loss_kind = loss_kind;
+  w.interval = interval;
+  w.ncp = ncp;
+  w.recorded = true;
+  return DUGKS_OK;
+}
+
+int dugks_adjoint_replay(dugks_ctx* ctx, double* dev_grad_force, double* dev_grad_f0, double* peak_bytes) {
+  CHECK_CTX();
+  if (!dev_grad_force) return fail(ctx, DUGKS_EARG, "null gradient buffer");
+  Ctx& c = ctx->c;
+  SweepPool& w = c.sweep;
+  if (!w.recorded) return fail(ctx, DUGKS_ESTATE, "no recorded forward sweep (dugks_forward_record)");
+  const size_t N = c.s.ncell();
+  const size_t nf = N * Q;
+  const size_t sb = c.store_bytes;
+  const bool lam32 = sweep_lam32(c);
+  const size_t lb = lam32 ? sizeof(float) : sizeof(double);
+  const int T = w.T, K = w.interval;
+  auto chk = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess)
+      return fail(ctx, e == hipErrorOutOfMemory ? DUGKS_ENOMEM : DUGKS_EHIP,
+                  std::string(what) + ": " + hipGetErrorString(e));
+    return DUGKS_OK;
+  };
+  int rc;
+  // segment states f^{n0+1} .. f^{n0+K-1}; f^{n0} is the checkpoint itself
+  if ((rc = chk(pool_states(c, w.seg, (size_t)(K - 1), nf * sb), "segment alloc"))) return rc;
+  if (w.lam_bytes < nf * lb) {
+    for (int q = 0; q < 2; q++) {
+      hipFree(w.lam[q]);
+      w.lam[q] = nullptr;
+    }
+    w.bytes -= 2.0 * w.lam_bytes;
+    c.bytes_allocated -= 2.0 * w.lam_bytes;
+    w.lam_bytes = 0;
+    for (int q = 0; q < 2; q++)
+      if ((rc = chk(c.alloc(&w.lam[q], nf * lb), "lambda alloc"))) return rc;
+    w.lam_bytes = nf * lb;
+    w.bytes += 2.0 * w.lam_bytes;
+  }
+  void *lam = w.lam[0], *lam2 = w.lam[1];
+  // dval[1]: loss values of the reverse sweep (discarded)
+  double* dval = c.red + reduction_scratch_doubles() + 9;
+  if (lam32) {
+    if ((rc = chk(hipMemsetAsync(lam, 0, nf * lb, c.stream), "clear"))) return rc;
+  } else {
+    if ((rc = chk(launch_fill(c, static_cast<double*>(lam), 0.0, (long long)nf), "clear"))) return rc;
+  }
+  if ((rc = chk(launch_fill(c, dev_grad_force, 0.0, (long long)(3 * N)), "clear"))) return rc;
   const double invT = 1.0 / T;
-  // Recompute segments from their checkpoints into seg[], then step back.
-  for (int q = ncp - 1; q >= 0; q--) {
-    const int n0 = q * interval, n1 = std::min(T, n0 + interval);
-    // seg[s] = f^{n0+s}
-    if ((rc = chk(hipMemcpyAsync(seg[0], cps[q], nf * sb, hipMemcpyDeviceToDevice, c.stream), "seg copy")))
-      return rc;
-    for (int s = 1; s < n1 - n0; s++) {
+  // Recompute each segment from its checkpoint, then step back through it.
+  for (int q = w.ncp - 1; q >= 0; q--) {
+    const int n0 = q * K, n1 = std::min(T, n0 + K);
+    auto state = [&](int n) -> const void* {  // f^n for n0 <= n <= n1
+      if (n == n0) return w.cps[q];
+      if (n < n1) return w.seg[n - n0 - 1];
+      return n1 == T ? w.fT : w.cps[q + 1];
+    };
+    for (int n = n0 + 1; n < n1; n++) {
       bool handled = false;
       hipError_t e = hipSuccess;
-      if (c.kernel != DUGKS_KERNEL_TWOPASS) e = launch_step_fused(c, seg[s - 1], seg[s], &handled);
-      if (e == hipSuccess && !handled) e = launch_step_twopass(c, seg[s - 1], seg[s]);
+      void* out = w.seg[n - n0 - 1];
+      if (c.kernel != DUGKS_KERNEL_TWOPASS) e = launch_step_fused(c, state(n - 1), out, &handled);
+      if (e == hipSuccess && !handled) e = launch_step_twopass(c, state(n - 1), out);
       if ((rc = chk(e, "segment recompute"))) return rc;
     }
     for (int n = n1 - 1; n >= n0; n--) {
-      // lam currently = dJ/df^{n+1}; add the loss term at step n+1
-      const void* fnext = (n + 1 < n1) ? seg[n + 1 - n0] : (n1 == T ? fT : cps[q + 1]);
-      // lam2 = (df^{n+1}/df^n)^T lam, about f^n = seg[n - n0]
+      // lam = dJ/df^{n+1}: add the loss term of step n+1, then
+      // lam2 = (df^{n+1}/df^n)^T lam about f^n
       if (lam32) {
         float *l = static_cast<float*>(lam), *l2 = static_cast<float*>(lam2);
-        if ((rc = chk(launch_loss_lam32(c, fnext, loss_kind, dval + 1, l, invT), "loss grad"))) return rc;
-        if ((rc = chk(launch_adjoint_lam32(c, seg[n - n0], l, l2, dev_grad_force), "adjoint"))) return rc;
+        if ((rc = chk(launch_loss_lam32(c, state(n + 1), w.loss_kind, dval + 1, l, invT), "loss grad"))) return rc;
+        if ((rc = chk(launch_adjoint_lam32(c, state(n), l, l2, dev_grad_force), "adjoint"))) return rc;
       } else {
         double *l = static_cast<double*>(lam), *l2 = static_cast<double*>(lam2);
-        if ((rc = chk(launch_loss(c, fnext, loss_kind, dval + 1, l, invT), "loss grad"))) return rc;
-        if ((rc = chk(launch_adjoint(c, seg[n - n0], l, l2, dev_grad_force), "adjoint"))) return rc;
+        if ((rc = chk(launch_loss(c, state(n + 1), w.loss_kind, dval + 1, l, invT), "loss grad"))) return rc;
+        if ((rc = chk(launch_adjoint(c, state(n), l, l2, dev_grad_force), "adjoint"))) return rc;
       }
       std::swap(lam, lam2);
     }
@@ -846,9 +906,35 @@ int dugks_forward_adjoint(dugks_ctx* ctx, int istep0, int T, int loss_kind, int 
                   "grad copy")))
       return rc;
   if ((rc = chk(hipStreamSynchronize(c.stream), "sync"))) return rc;
-  if (peak_bytes) *peak_bytes = bytes;
-  cleanup();
+  if (peak_bytes) {
+    // device memory in use at the end of the sweep, when every buffer it
+    // needs (pool, adjoint face records, state) is allocated
+    size_t free_b = 0, total_b = 0;
+    if ((rc = chk(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo"))) return rc;
+    peak_bytes[0] = (double)(total_b - free_b);
+  }
   return DUGKS_OK;
+}
+
+int dugks_release_sweep(dugks_ctx* ctx) {
+  CHECK_CTX();
+  release_sweep(ctx->c);
+  return DUGKS_OK;
+}
+
+int dugks_sweep_bytes(dugks_ctx* ctx, double* bytes) {
+  if (!ctx || !bytes) return DUGKS_EARG;
+  *bytes = ctx->c.sweep.bytes;
+  return DUGKS_OK;
+}
+
+int dugks_forward_adjoint(dugks_ctx* ctx, int istep0, int T, int loss_kind, int interval, double* J,
+                          double* dev_grad_force, double* dev_grad_f0, double* peak_bytes) {
+  CHECK_CTX();
+  if (!dev_grad_force) return fail(ctx, DUGKS_EARG, "bad forward_adjoint arguments");
+  int rc = dugks_forward_record(ctx, istep0, T, loss_kind, interval, J);
+  if (rc) return rc;
+  return dugks_adjoint_replay(ctx, dev_grad_force, dev_grad_f0, peak_bytes);
 }
 
 // ---- host-only helpers ----

@@ -149,6 +149,44 @@ __device__ __forceinline__ void st_state(const StateBuf<Store>& b, unsigned vb, 
     b.put((Store)v, vb, sb);
 }
 
+// Sum over the 64 lanes of a wave by an xor butterfly: a fixed order, and
+// every lane ends with the same bits (each level adds the same two values).
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+  for (int m = 32; m >= 1; m >>= 1) v = fmin(v, __shfl_xor(v, m, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
+// Block reduction of NV values per thread (block of NT threads, a multiple
+// of 64): wave butterflies, then the wave results in wave order.  Op 0 sums,
+// 1 takes the minimum, 2 the maximum (per value, from ops[q]).  The result
+// is valid in thread 0.  Deterministic for a fixed block size.
+template <int NV, int NT>
+__device__ __forceinline__ void block_reduce(double (&v)[NV], const int (&ops)[NV]) {
+  constexpr int NW = NT / 64;
+  __shared__ double part[NV][NW];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int q = 0; q < NV; q++) {
+    v[q] = ops[q] == 0 ? wave_sum(v[q]) : (ops[q] == 1 ? wave_min(v[q]) : wave_max(v[q]));
+    if (lane == 0) part[q][w] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int q = 0; q < NV; q++) {
+      double a = part[q][0];
+      for (int k = 1; k < NW; k++)
+        a = ops[q] == 0 ? a + part[q][k] : (ops[q] == 1 ? fmin(a, part[q][k]) : fmax(a, part[q][k]));
+      v[q] = a;
+    }
+}
+
 // 1/x: correctly rounded in FP64; the hardware reciprocal (1 ulp) in FP32.
 template <class Real>
 __device__ __forceinline__ Real rcp_real(Real x) {
@@ -172,43 +210,113 @@ __device__ __forceinline__ float tabv<float>(const KParams& P, int a, int idx) {
   return P.tabf[a][idx];
 }
 
+// c_ip . (x, y, z) over the non-zero components only.  IEEE arithmetic
+// cannot drop a product with a zero component (0 * inf, the sign of zero),
+// so writing the dot product out would cost a multiply-add per zero
+// component.  Zero for the rest population.
+template <int ip, class Real>
+__device__ __forceinline__ Real cdot(Real x, Real y, Real z) {
+  constexpr int cx = CX[ip], cy = CY[ip], cz = CZ[ip];
+  if constexpr (cx != 0) {
+    Real s = cx > 0 ? x : -x;
+    if constexpr (cy != 0) s = cy > 0 ? s + y : s - y;
+    if constexpr (cz != 0) s = cz > 0 ? s + z : s - z;
+    return s;
+  } else if constexpr (cy != 0) {
+    Real s = cy > 0 ? y : -y;
+    if constexpr (cz != 0) s = cz > 0 ? s + z : s - z;
+    return s;
+  } else if constexpr (cz != 0) {
+    return cz > 0 ? z : -z;
+  } else {
+    return Real(0);
+  }
+}
+
+// a + c_ip . (x, y, z), the same way (a for the rest population).
+template <int ip, class Real>
+__device__ __forceinline__ Real plus_cdot(Real a, Real x, Real y, Real z) {
+  if constexpr (ip == 0)
+    return a;
+  else
+    return a + cdot<ip>(x, y, z);
+}
+
+// Weight class of a population: 0 rest (1/3), 1 axis (1/18), 2 diagonal (1/36).
+__device__ constexpr int wclass(int ip) { return ip == 0 ? 0 : (ip <= 6 ? 1 : 2); }
+__device__ constexpr double wclass_weight(int k) { return k == 0 ? W0 : (k == 1 ? W1 : W2); }
+
+// Equilibrium-with-force term of the centre stage and the face collision,
+//   G_i = (P + Kf (c_i - u).F / RT) feq_i(rho, u),
+// feq_i = w_i rho (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u.u), held as the
+// per-weight-class products w rho (P - 3 Kf u.F) and 3 Kf w rho, so that
+// each population costs c.u, c.F and four fused multiply-adds.
+template <class Real>
+struct EqTerm {
+  Real ux, uy, uz, base, A[3], B[3];
+  __device__ __forceinline__ EqTerm(Real rho, Real ux_, Real uy_, Real uz_, const Real F[3], Real P, Real Kf)
+      : ux(ux_), uy(uy_), uz(uz_) {
+    base = Real(1) - Real(1.5) * (ux * ux + uy * uy + uz * uz);  // 1 - u.u/(2RT)
+    const Real uF = ux * F[0] + uy * F[1] + uz * F[2];
+    const Real P0 = P - Real(3) * Kf * uF;
+    const Real K3 = Real(3) * Kf;
+    for (int k = 0; k < 3; k++) {
+      const Real wr = Real(wclass_weight(k)) * rho;
+      A[k] = wr * P0;
+      B[k] = wr * K3;
+    }
+  }
+  // G_ip given c_ip.F (cF, unused for the rest population)
+  template <int ip>
+  __device__ __forceinline__ Real term(Real cF) const {
+    if constexpr (ip == 0) {
+      return A[0] * base;
+    } else {
+      const Real cu = cdot<ip>(ux, uy, uz);
+      const Real e = cu * (cu * Real(4.5) + Real(3)) + base;
+      return (cF * B[wclass(ip)] + A[wclass(ip)]) * e;
+    }
+  }
+  // s f + G_ip
+  template <int ip>
+  __device__ __forceinline__ Real relax(Real s, Real f, Real cF) const {
+    if constexpr (ip == 0) {
+      return A[0] * base + s * f;
+    } else {
+      const Real cu = cdot<ip>(ux, uy, uz);
+      const Real e = cu * (cu * Real(4.5) + Real(3)) + base;
+      return (cF * B[wclass(ip)] + A[wclass(ip)]) * e + s * f;
+    }
+  }
+};
+
 // Centre-stage f-bar+ of one cell (calc_fb1 of the reference, src/dugks.cu:1131-1196):
 //   rho = sum f, u = j/rho + h F, feq_i(rho, u),
 //   fb_i = bw1 f_i + (bw2 + bw0 (c_i - u).F / RT) feq_i.
-// Written with the +/- pairing of opposite populations, which only reorders
-// the arithmetic.  This form takes rho and j = sum c f of the cell (from
-// cell_moments below).
+// This form takes rho and j = sum c f of the cell (from cell_moments below).
 template <class Real>
 __device__ __forceinline__ void centre_fbar(const Real (&f)[Q], Real rho, const Real (&j)[3], const Real F[3],
                                             Real hdt, Real bw0, Real bw1, Real bw2, Real (&fb)[Q]) {
   const Real inv = rcp_real(rho);
-  const Real ux = j[0] * inv + hdt * F[0];
-  const Real uy = j[1] * inv + hdt * F[1];
-  const Real uz = j[2] * inv + hdt * F[2];
-  const Real uu = ux * ux + uy * uy + uz * uz;
-  const Real base = Real(1) - Real(1.5) * uu;  // 1 - u.u/(2RT)
-  const Real uF = ux * F[0] + uy * F[1] + uz * F[2];
-  const Real P0 = bw2 - Real(3) * bw0 * uF;    // bw2 + bw0 (-u.F)/RT
+  const EqTerm<Real> eq(rho, j[0] * inv + hdt * F[0], j[1] * inv + hdt * F[1], j[2] * inv + hdt * F[2], F, bw2,
+                        bw0);
   static_for<Q>([&](auto I) {
     constexpr int ip = decltype(I)::value;
-    const Real cu = Real(CX[ip]) * ux + Real(CY[ip]) * uy + Real(CZ[ip]) * uz;
-    const Real cF = Real(CX[ip]) * F[0] + Real(CY[ip]) * F[1] + Real(CZ[ip]) * F[2];
-    const Real feq = Real(wgt(ip)) * rho * (base + Real(3) * cu + Real(4.5) * cu * cu);
-    fb[ip] = bw1 * f[ip] + (P0 + Real(3) * bw0 * cF) * feq;
+    fb[ip] = eq.template relax<ip>(bw1, f[ip], cdot<ip>(F[0], F[1], F[2]));
   });
 }
 
-// rho = sum_i f_i and j = sum_i c_i f_i of one cell, summed in population order.
+// rho = sum_i f_i and j = sum_i c_i f_i of one cell, summed in population
+// order (each sum starts from its first term).
 template <class Real>
 __device__ __forceinline__ void cell_moments(const Real (&ff)[Q], Real& rho, Real (&j)[3]) {
-  rho = 0;
-  j[0] = j[1] = j[2] = 0;
   static_for<Q>([&](auto I) {
     constexpr int ip = decltype(I)::value;
-    rho += ff[ip];
-    if constexpr (CX[ip] != 0) j[0] += CX[ip] > 0 ? ff[ip] : -ff[ip];
-    if constexpr (CY[ip] != 0) j[1] += CY[ip] > 0 ? ff[ip] : -ff[ip];
-    if constexpr (CZ[ip] != 0) j[2] += CZ[ip] > 0 ? ff[ip] : -ff[ip];
+    if constexpr (ip == 0) rho = ff[0]; else rho += ff[ip];
+    // first populations with a non-zero x, y, z component: 1, 3, 5 (c = +1)
+    if constexpr (ip == 1) j[0] = ff[1]; else if constexpr (CX[ip] != 0) j[0] += CX[ip] > 0 ? ff[ip] : -ff[ip];
+    if constexpr (ip == 3) j[1] = ff[3]; else if constexpr (CY[ip] != 0) j[1] += CY[ip] > 0 ? ff[ip] : -ff[ip];
+    if constexpr (ip == 5) j[2] = ff[5]; else if constexpr (CZ[ip] != 0) j[2] += CZ[ip] > 0 ? ff[ip] : -ff[ip];
   });
 }
 

@@ -88,10 +88,6 @@ hipError_t launch_adjoint(Ctx& c, const void* f, const double* lam_next, double*
 hipError_t launch_adjoint_lam32(Ctx& c, const void* f, const float* lam_next, float* lam_prev,
                                 double* grad_force);
 bool adjoint_lam32_supported(const Ctx& c);
-// One-pass adjoint step (adjoint_fused.hip); handled = false when it does not
-// cover the configuration.
-hipError_t launch_adjoint_onepass(Ctx& c, const void* f, const double* lam_next, double* lam_prev,
-                                  double* grad_force, bool* handled);
 // Loss value into dev_value (added to it when value_acc) and, when dev_grad
 // is set, its gradient dJ/df written into dev_grad (grad_acc == 0) or added
 // to it times grad_acc.
@@ -104,6 +100,26 @@ hipError_t launch_loss_lam32(Ctx& c, const void* f, int kind, double* dev_value,
 hipError_t launch_widen(Ctx& c, const float* src, double* dst, long long n);
 hipError_t launch_axpy(Ctx& c, double a, const double* x, double* y, long long n);
 
+// dst[i] = v for i < n (a fill kernel: safe inside captured graphs, where
+// memset nodes are avoided).
+hipError_t launch_fill(Ctx& c, double* dst, double v, long long n);
+
+// Checkpoint pool of the sqrt(T) forward-adjoint sweep, kept on the context
+// between calls (dugks_forward_record fills it, dugks_adjoint_replay reads
+// it): checkpoints cps[q] = f^{q K} (K = interval), segment states seg[s] =
+// f^{n0+1+s} of the segment being replayed, the final state fT, and two
+// adjoint buffers.  Buffers are allocated on demand and reused.
+struct SweepPool {
+  std::vector<void*> cps, seg;
+  void* fT = nullptr;
+  void* lam[2] = {nullptr, nullptr};
+  size_t lam_bytes = 0;
+  double bytes = 0;  // device bytes held by the pool
+  // the recorded forward sweep
+  bool recorded = false;
+  int istep0 = 0, T = 0, loss_kind = 0, interval = 0, ncp = 0;
+};
+
 // Library context.
 struct Ctx {
   Setup s;
@@ -113,7 +129,6 @@ struct Ctx {
   bool own_stream = false;
   int precision = 0;
   int kernel = 0;
-  int adjoint_path = 0;       // DUGKS_ADJOINT_AUTO, _SPLIT or _ONEPASS
   size_t store_bytes = 8;     // bytes per stored f value
   void* f[2] = {nullptr, nullptr};
   int cur = 0;                // f[cur] holds the current state
@@ -140,6 +155,7 @@ struct Ctx {
   int fused_tz = 0;
   double* flow_partial = nullptr;
   size_t flow_partial_len = 0;
+  SweepPool sweep;
 
   hipError_t alloc(void** p, size_t bytes) {
     hipError_t e = hipMalloc(p, bytes);

@@ -1,8 +1,8 @@
-// Building blocks shared by the fused step kernel and the one-pass adjoint
-// kernel: per-axis direction sets, f-bar+ of a (possibly wall-ghost) column,
-// and the face reconstruction record with its collision and the transpose
-// of that collision.  Included inside namespace dugks { namespace { ... } }
-// of each kernel file.
+// Building blocks of the fused step kernel and its adjoint-face mode:
+// per-axis direction sets, f-bar+ of a (possibly wall-ghost) column, and the
+// face reconstruction record with its collision and the transpose of that
+// collision.  Included inside namespace dugks { namespace { ... } } of the
+// kernel files.
 #pragma once
 
 // populations with a non-zero component along each axis
@@ -131,7 +131,7 @@ __device__ __forceinline__ void expand_cotangents(const Real (&mu)[10], Real Ac,
   static_for<Q>([&](auto I) {
     constexpr int ip = decltype(I)::value;
     constexpr int k = cross_index<A>(ip);
-    Real v = Ac + B[0] * Real(CX[ip]) + B[1] * Real(CY[ip]) + B[2] * Real(CZ[ip]);
+    Real v = plus_cdot<ip>(Ac, B[0], B[1], B[2]);
     if constexpr (k >= 0) v += fw1 * mu[k];
     post[ip] = v;
   });
@@ -152,9 +152,7 @@ __device__ __forceinline__ void compact_record(const Real (&mu)[10], Real Ac, co
   static_for<10>([&](auto K) {
     constexpr int k = decltype(K)::value;
     constexpr int ip = dirs<A>(k);
-    Real v = Ac + B[0] * Real(CX[ip]) + B[1] * Real(CY[ip]) + B[2] * Real(CZ[ip]);
-    v += fw1 * mu[k];
-    rec[k] = v;
+    rec[k] = plus_cdot<ip>(Ac, B[0], B[1], B[2]) + fw1 * mu[k];
   });
   rec[10] = Ac;
   rec[11] = B[tangential<A>(0)];
@@ -178,29 +176,61 @@ __device__ __forceinline__ void expand_record(const Real (&rec)[kCompact], bool 
     if constexpr (k >= 0)
       post[ip] = rec[k];
     else
-      post[ip] = Ac + B[0] * Real(CX[ip]) + B[1] * Real(CY[ip]) + B[2] * Real(CZ[ip]);
+      post[ip] = plus_cdot<ip>(Ac, B[0], B[1], B[2]);
   });
   bounce_transpose<Real, A>(post, low_wall, high_wall, lam);
 }
 
+// First population that does not cross a face normal to axis A and has a
+// non-zero component b (b != A); the moments of a face start from it.
+template <int A>
+__device__ constexpr int first_tangential(int b) {
+  for (int ip = 0; ip < Q; ip++)
+    if (cv(A, ip) == 0 && cv(b, ip) != 0) return ip;
+  return -1;
+}
+
 // Reconstructed populations of one face normal to axis A, held as the ten
 // values that cross the face plus the partial moments of the nine that do
-// not (those only enter rho and j of the face collision).
+// not (those only enter rho and j of the face collision).  The populations
+// are put in order 0..18 (the rest population first); every moment starts
+// from its first term, so no sum adds to a zero.
 template <class Real, int A>
 struct Face {
   Real cross[10];
-  Real rho = 0, jx = 0, jy = 0, jz = 0;
+  Real rho, j[3];
   template <int ip>
   __device__ __forceinline__ void put(Real v) {
     constexpr int k = cross_index<A>(ip);
     if constexpr (k >= 0) {
       cross[k] = v;
     } else {
-      rho += v;
-      if constexpr (CX[ip] != 0) jx += CX[ip] > 0 ? v : -v;
-      if constexpr (CY[ip] != 0) jy += CY[ip] > 0 ? v : -v;
-      if constexpr (CZ[ip] != 0) jz += CZ[ip] > 0 ? v : -v;
+      if constexpr (ip == 0) rho = v; else rho += v;
+      static_for<3>([&](auto B) {
+        constexpr int b = decltype(B)::value;
+        if constexpr (b != A && cv(b, ip) != 0) {
+          const Real s = cv(b, ip) > 0 ? v : -v;
+          if constexpr (ip == first_tangential<A>(b)) j[b] = s; else j[b] += s;
+        }
+      });
     }
+  }
+  // rho and j of the face: the partial moments plus the crossing populations
+  // (j_A starts from the first crossing population, c_A = +1)
+  __device__ __forceinline__ void moments() {
+    static_for<10>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      constexpr int ip = dirs<A>(k);
+      const Real v = cross[k];
+      rho += v;
+      static_for<3>([&](auto B) {
+        constexpr int b = decltype(B)::value;
+        if constexpr (cv(b, ip) != 0) {
+          const Real s = cv(b, ip) > 0 ? v : -v;
+          if constexpr (b == A && k == 0) j[b] = s; else j[b] += s;
+        }
+      });
+    });
   }
   // Half-way bounce-back: populations entering the domain through a wall
   // take the value of their opposite (src/dugks.cu:807-823).
@@ -225,31 +255,15 @@ struct Face {
   // for the crossing populations.
   __device__ __forceinline__ void collide(const Real F[3], Real hdt, Real fw0, Real fw1, Real fw2,
                                           Real (&out)[10]) {
-    static_for<10>([&](auto K) {
-      constexpr int k = decltype(K)::value;
-      constexpr int ip = dirs<A>(k);
-      const Real v = cross[k];
-      rho += v;
-      if constexpr (CX[ip] != 0) jx += CX[ip] > 0 ? v : -v;
-      if constexpr (CY[ip] != 0) jy += CY[ip] > 0 ? v : -v;
-      if constexpr (CZ[ip] != 0) jz += CZ[ip] > 0 ? v : -v;
-    });
+    moments();
     const Real inv = rcp_real(rho);
     const Real hh = Real(0.5) * hdt;
-    const Real ux = jx * inv + hh * F[0];
-    const Real uy = jy * inv + hh * F[1];
-    const Real uz = jz * inv + hh * F[2];
-    const Real uu = ux * ux + uy * uy + uz * uz;
-    const Real base = Real(1) - Real(1.5) * uu;
-    const Real uF = ux * F[0] + uy * F[1] + uz * F[2];
-    const Real P0 = fw2 - Real(3) * fw0 * uF;
+    const EqTerm<Real> eq(rho, j[0] * inv + hh * F[0], j[1] * inv + hh * F[1], j[2] * inv + hh * F[2], F, fw2,
+                          fw0);
     static_for<10>([&](auto K) {
       constexpr int k = decltype(K)::value;
       constexpr int ip = dirs<A>(k);
-      const Real cu = Real(CX[ip]) * ux + Real(CY[ip]) * uy + Real(CZ[ip]) * uz;
-      const Real cF = Real(CX[ip]) * F[0] + Real(CY[ip]) * F[1] + Real(CZ[ip]) * F[2];
-      const Real feq = Real(wgt(ip)) * rho * (base + Real(3) * cu + Real(4.5) * cu * cu);
-      out[k] = fw1 * cross[k] + (P0 + Real(3) * fw0 * cF) * feq;
+      out[k] = eq.template relax<ip>(fw1, cross[k], cdot<ip>(F[0], F[1], F[2]));
     });
   }
   // The transposed collision up to its equilibrium coefficients: the
@@ -257,18 +271,10 @@ struct Face {
   // for the populations that do not cross), before the bounce transpose.
   __device__ __forceinline__ void adjoint_coeffs(const Real F[3], Real hdt, Real fw0, Real fw2,
                                                  const Real (&mu)[10], Real& Ac, Real (&B)[3]) {
-    static_for<10>([&](auto K) {
-      constexpr int k = decltype(K)::value;
-      constexpr int ip = dirs<A>(k);
-      const Real v = cross[k];
-      rho += v;
-      if constexpr (CX[ip] != 0) jx += CX[ip] > 0 ? v : -v;
-      if constexpr (CY[ip] != 0) jy += CY[ip] > 0 ? v : -v;
-      if constexpr (CZ[ip] != 0) jz += CZ[ip] > 0 ? v : -v;
-    });
+    moments();
     const Real inv = rcp_real(rho);
     const Real hh = Real(0.5) * hdt;
-    const Real ut[3] = {jx * inv, jy * inv, jz * inv};
+    const Real ut[3] = {j[0] * inv, j[1] * inv, j[2] * inv};
     const Real u[3] = {ut[0] + hh * F[0], ut[1] + hh * F[1], ut[2] + hh * F[2]};
     Real lq[Q];
     static_for<Q>([&](auto I) {

@@ -4,7 +4,10 @@
 //   diag.dat, profiles   src/output.cu:223-334
 #include "io.h"
 
+#include <charconv>
+#include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <vector>
 
@@ -150,12 +153,52 @@ void finish_statistics(const Setup& s, const double* sums, double* profiles, dou
   }
 }
 
+// A double as Python's repr() writes it (the VELMEAN.dat rows are Python
+// f-strings): the shortest digit string that reads back to the same double,
+// in positional notation with at least one fractional digit ("1.0",
+// "0.0001") when the decimal exponent e of the leading digit satisfies
+// -4 <= e < 16, else as d[.ddd]e+XX with at least two exponent digits
+// ("1e-05", "1.5e+16").  The shortest round-trip digits come from
+// std::to_chars in scientific form.
+std::string py_repr(double v) {
+  if (std::isnan(v)) return "nan";
+  if (std::isinf(v)) return v < 0 ? "-inf" : "inf";
+  char buf[64];
+  auto r = std::to_chars(buf, buf + sizeof buf, v, std::chars_format::scientific);
+  std::string s(buf, r.ptr);
+  std::string sign;
+  if (s[0] == '-') {
+    sign = "-";
+    s = s.substr(1);
+  }
+  const size_t epos = s.find('e');
+  const int e = std::stoi(s.substr(epos + 1));
+  std::string digits;
+  for (size_t q = 0; q < epos; q++)
+    if (s[q] != '.') digits += s[q];
+  const int nd = (int)digits.size();
+  if (e < -4 || e >= 16) {
+    std::string m = digits.substr(0, 1);
+    if (nd > 1) m += "." + digits.substr(1);
+    char eb[16];
+    std::snprintf(eb, sizeof eb, "e%c%02d", e < 0 ? '-' : '+', e < 0 ? -e : e);
+    return sign + m + eb;
+  }
+  const int decpt = e + 1;  // digits before the decimal point
+  if (decpt <= 0) return sign + "0." + std::string((size_t)-decpt, '0') + digits;
+  if (decpt < nd) return sign + digits.substr(0, (size_t)decpt) + "." + digits.substr((size_t)decpt);
+  return sign + digits + std::string((size_t)(decpt - nd), '0') + ".0";
+}
+
 // Time average and half-channel symmetrisation of profiles-2.dat into
 // VELMEAN.dat (examples/channel/ProcessProfiles.py:17-79 of the reference):
 // the blocks whose step exceeds ignore_before are averaged; the mean and the
 // rms columns are symmetrised as (a + reverse a) / 2, the shear stress as
 // (s - reverse s) / 2.  Output rows: index (1-based), mean, stress, and the
-// three variance columns.
+// three variance columns, each written as Python's repr() writes it.  The
+// step of each block is the one in the file (the reference script counts
+// blocks on its params schedule, which gives the same steps for a file
+// written on that schedule).
 void velmean(const std::string& in_path, int n, int ignore_before, int mean_col, int stress_col,
              const std::string& out_path) {
   FILE* f = std::fopen(in_path.c_str(), "r");
@@ -192,9 +235,11 @@ void velmean(const std::string& in_path, int n, int ignore_before, int mean_col,
   for (auto& v : var) sym(v, 1.0);
   FILE* g = std::fopen(out_path.c_str(), "w");
   if (!g) throw ConfigError(DUGKS_EIO, "cannot write " + out_path);
-  for (int k = 0; k < n; k++)
-    std::fprintf(g, "%.1f %.17g %.17g %.17g %.17g %.17g\n", (double)(k + 1), mean[k], stress[k], var[0][k],
-                 var[1][k], var[2][k]);
+  for (int k = 0; k < n; k++) {
+    const std::string row = py_repr((double)(k + 1)) + " " + py_repr(mean[k]) + " " + py_repr(stress[k]) + " " +
+                            py_repr(var[0][k]) + " " + py_repr(var[1][k]) + " " + py_repr(var[2][k]) + "\n";
+    std::fputs(row.c_str(), g);
+  }
   std::fclose(g);
 }
 

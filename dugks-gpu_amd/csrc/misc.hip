@@ -59,19 +59,20 @@ __global__ void __launch_bounds__(256) k_import(KParams P, const double* __restr
   Codec<Store>::template store<double>(f, t, src[t], P, ip);
 }
 
+// Ops of block_reduce: NV sums.
+template <int NV>
+struct AllSums {
+  int op[NV];
+  __device__ constexpr AllSums() : op() {}
+};
+
 // Fixed-order block reduction of NV doubles per thread into out[blockIdx*NV..].
 template <int NV>
 __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* out) {
-  __shared__ double sh[NV][kRedThreads];
-  for (int q = 0; q < NV; q++) sh[q][threadIdx.x] = v[q];
-  __syncthreads();
-  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s)
-      for (int q = 0; q < NV; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
-    __syncthreads();
-  }
+  constexpr AllSums<NV> sums;
+  block_reduce<NV, kRedThreads>(v, sums.op);
   if (threadIdx.x == 0)
-    for (int q = 0; q < NV; q++) out[blockIdx.x * NV + q] = sh[q][0];
+    for (int q = 0; q < NV; q++) out[blockIdx.x * NV + q] = v[q];
 }
 
 // Sums: mass, momentum (3), volume-weighted macro velocity (3), volume.
@@ -114,16 +115,10 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce_final(const double* part
   for (int q = 0; q < NV; q++) acc[q] = 0;
   for (int b = threadIdx.x; b < nparts; b += kRedThreads)
     for (int q = 0; q < NV; q++) acc[q] += partial[b * NV + q];
-  __shared__ double sh[NV][kRedThreads];
-  for (int q = 0; q < NV; q++) sh[q][threadIdx.x] = acc[q];
-  __syncthreads();
-  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s)
-      for (int q = 0; q < NV; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
-    __syncthreads();
-  }
+  constexpr AllSums<NV> sums;
+  block_reduce<NV, kRedThreads>(acc, sums.op);
   if (threadIdx.x == 0)
-    for (int q = 0; q < NV; q++) out[q] = (accumulate ? out[q] : 0.0) + sh[q][0];
+    for (int q = 0; q < NV; q++) out[q] = (accumulate ? out[q] : 0.0) + acc[q];
 }
 
 // Constant-flow-rate controller (single thread).  famb_dev layout:
@@ -284,6 +279,11 @@ __global__ void __launch_bounds__(256) k_axpy(double a, const double* __restrict
   if (t < n) y[t] += a * x[t];
 }
 
+__global__ void __launch_bounds__(256) k_fill(double* __restrict__ y, double v, long long n) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) y[i] = v;
+}
+
 __global__ void __launch_bounds__(256) k_widen(const float* __restrict__ x, double* __restrict__ y, long long n) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) y[t] = (double)x[t];
@@ -336,28 +336,24 @@ __global__ void __launch_bounds__(kRedThreads) k_stats(KParams P, const Store* _
       mx[q] = fmax(mx[q], g[q]);
     }
   }
-  __shared__ double sh[kStatSums + 8][kRedThreads];
-  for (int q = 0; q < kStatSums; q++) sh[q][threadIdx.x] = acc[q];
+  // sums, then 4 minima, then 4 maxima
+  double all[kStatSums + 8];
+  int ops[kStatSums + 8];
+  for (int q = 0; q < kStatSums; q++) {
+    all[q] = acc[q];
+    ops[q] = 0;
+  }
   for (int q = 0; q < 4; q++) {
-    sh[kStatSums + q][threadIdx.x] = mn[q];
-    sh[kStatSums + 4 + q][threadIdx.x] = mx[q];
+    all[kStatSums + q] = mn[q];
+    ops[kStatSums + q] = 1;
+    all[kStatSums + 4 + q] = mx[q];
+    ops[kStatSums + 4 + q] = 2;
   }
-  __syncthreads();
-  for (int st = kRedThreads / 2; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) {
-      for (int q = 0; q < kStatSums; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + st];
-      for (int q = 0; q < 4; q++) {
-        sh[kStatSums + q][threadIdx.x] = fmin(sh[kStatSums + q][threadIdx.x], sh[kStatSums + q][threadIdx.x + st]);
-        sh[kStatSums + 4 + q][threadIdx.x] =
-            fmax(sh[kStatSums + 4 + q][threadIdx.x], sh[kStatSums + 4 + q][threadIdx.x + st]);
-      }
-    }
-    __syncthreads();
-  }
+  block_reduce<kStatSums + 8, kRedThreads>(all, ops);
   if (threadIdx.x == 0) {
     const long long slot = (long long)m * kStatBlocks + b;
-    for (int q = 0; q < kStatSums; q++) part[slot * kStatSums + q] = sh[q][0];
-    for (int q = 0; q < 8; q++) mm[slot * 8 + q] = sh[kStatSums + q][0];
+    for (int q = 0; q < kStatSums; q++) part[slot * kStatSums + q] = all[q];
+    for (int q = 0; q < 8; q++) mm[slot * 8 + q] = all[kStatSums + q];
   }
 }
 
@@ -514,6 +510,12 @@ hipError_t launch_loss_lam32(Ctx& c, const void* f, int kind, double* dev_value,
 
 hipError_t launch_axpy(Ctx& c, double a, const double* x, double* y, long long n) {
   hipLaunchKernelGGL(k_axpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, a, x, y, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(Ctx& c, double* dst, double v, long long n) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, dst, v, n);
   return hipGetLastError();
 }
 

@@ -53,7 +53,8 @@ struct AdjFaceIO {
 // cotangents of its 19 reconstructed values, driven by lam' (adj.lam), are
 // written to adj.faces; nothing else is written.
 template <class Real, class Store, int TX, int TY, bool ADJ = false, bool COMPACT = false, class LamT = double>
-__global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64) k_fused(
+// FP32 arithmetic: two blocks per CU (at most 256 registers per lane)
+__global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64, sizeof(Real) == 4 ? 2 : 1) k_fused(
     KParams P, const Store* __restrict__ fin,
                                                           Store* __restrict__ fout, int TZ,
                                                           double* __restrict__ partial, AdjFaceIO adj) {
@@ -240,12 +241,55 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
   };
   const long long e0 = P.e[0], e1 = P.e[1];
 
+  // Columns inside the domain (tile and periodic halo) compute f-bar+ from f
+  // of their own column; wall ghost columns extrapolate it (fbar_column).
+  const bool own = active && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+  // f of the own column at plane k
+  auto load_own = [&](int k, Real (&ff)[Q]) {
+    const unsigned vb = (((unsigned)k * ny + gj) * nx + gi) * (unsigned)sizeof(Store);
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      ff[ip] = ld_state<Real, Store>(bin, vb, ip * Nb, P, ip);
+    });
+  };
+  // body force at plane k of the own column
+  auto force_own = [&](int k, Real (&F)[3]) {
+    F[0] = Famb[0];
+    F[1] = Famb[1];
+    F[2] = Famb[2];
+    if (P.fcell) {
+      const unsigned cc = ((unsigned)k * ny + gj) * nx + gi;
+      F[0] += (Real)P.fcell[cc];
+      F[1] += (Real)P.fcell[P.N + cc];
+      F[2] += (Real)P.fcell[2 * P.N + cc];
+    }
+  };
+  // The update of plane p needs 4/3 f-bar+(p) - 1/3 f(p); it is formed when
+  // f(p) arrives for f-bar+(p), so f is read once per step: gc holds it for
+  // the current plane, gn for the next.
+  Real gc[Q], gn[Q];
+  auto form_g = [&](const Real (&fb)[Q], const Real (&ff)[Q], Real (&g)[Q]) {
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      g[ip] = fb[ip] * Real(4.0 / 3.0) - ff[ip] * Real(1.0 / 3.0);
+    });
+  };
+
   // ---- pre-loop: f-bar at z0-1 and z0, and the -z face of plane z0 ----
   {
     const int km = (z0 - 1 + nz) % nz;
     if (active) {
       fbar_column<Real, Store>(P, bin, Nb, gi, gj, km, Famb, fbn);
-      fbar_column<Real, Store>(P, bin, Nb, gi, gj, z0, Famb, fbc);
+      if (!ADJ && own) {
+        Real ff[Q], F[3], rho, jm[3];
+        load_own(z0, ff);
+        force_own(z0, F);
+        cell_moments<Real>(ff, rho, jm);
+        centre_fbar<Real>(ff, rho, jm, F, hdt, (Real)P.bw0, (Real)P.bw1, (Real)P.bw2, fbc);
+        if constexpr (!ADJ) form_g(fbc, ff, gc);
+      } else {
+        fbar_column<Real, Store>(P, bin, Nb, gi, gj, z0, Famb, fbc);
+      }
       static_for<5>([&](auto K) {
         constexpr int k = decltype(K)::value;
         fbm[k] = fbn[ZP[k]];
@@ -259,20 +303,12 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
     __syncthreads();
   }
 
-  // f of the own column two planes ahead (non-ghost columns): the loads are
-  // issued one iteration before their f-bar needs them (FP16 storage gains
-  // nothing from it and loads at use)
+  // f of the own column one plane ahead of its f-bar: the loads are issued
+  // one iteration before their f-bar needs them (FP16 storage gains nothing
+  // from it and loads at use)
   constexpr bool kPrefetch = !std::is_same<Store, __half>::value;
-  const bool own = kPrefetch && active && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
   Real fpre[Q];
-  auto prefetch = [&](int k) {
-    const unsigned vb = (((unsigned)k * ny + gj) * nx + gi) * (unsigned)sizeof(Store);
-    static_for<Q>([&](auto I) {
-      constexpr int ip = decltype(I)::value;
-      fpre[ip] = ld_state<Real, Store>(bin, vb, ip * Nb, P, ip);
-    });
-  };
-  if (own) prefetch((z0 + 1) % nz);
+  if (kPrefetch && own) load_own((z0 + 1) % nz, fpre);
   // compact adjoint-face records with FP32 storage: rho and j of every cell
   // for the gather, written by the interior lanes from the prefetched f.
   // A chunk writes its planes z0+1 .. zend (mod nz), so the chunks cover
@@ -290,20 +326,19 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
     const int kn = (p + 1) % nz;
     if constexpr (ADJ) load_lam(kn, lamn);
     if (own) {
-      Real F[3] = {Famb[0], Famb[1], Famb[2]};
-      if (P.fcell) {
-        const unsigned cc = ((unsigned)kn * ny + gj) * nx + gi;
-        F[0] += (Real)P.fcell[cc];
-        F[1] += (Real)P.fcell[P.N + cc];
-        F[2] += (Real)P.fcell[2 * P.N + cc];
-      }
+      if constexpr (!kPrefetch) load_own(kn, fpre);
+      Real F[3];
+      force_own(kn, F);
       Real rho, jm[3];
       cell_moments<Real>(fpre, rho, jm);
       if constexpr (ADJ && COMPACT) {
         if (interior) store_moments(kn, rho, jm);
       }
       centre_fbar<Real>(fpre, rho, jm, F, hdt, (Real)P.bw0, (Real)P.bw1, (Real)P.bw2, fbn);
-      if (p + 1 < zend) prefetch((p + 2) % nz);
+      if constexpr (!ADJ) {
+        if (interior) form_g(fbn, fpre, gn);
+      }
+      if (kPrefetch && p + 1 < zend) load_own((p + 2) % nz, fpre);
     } else if (active) {
       fbar_column<Real, Store>(P, bin, Nb, gi, gj, kn, Famb, fbn);
     }
@@ -333,17 +368,6 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
     }
     __syncthreads();
     // ---- phase 2: the faces of plane p (-x, -y from Z; +z from Zn) ----
-    // f~(p) for the update, loaded now so that the face work hides the wait
-    Real fo[Q];
-    if constexpr (!ADJ) {
-      if (interior) {
-        const unsigned vb0 = (((unsigned)p * ny + gj) * nx + gi) * (unsigned)sizeof(Store);
-        static_for<Q>([&](auto I) {
-          constexpr int ip = decltype(I)::value;
-          fo[ip] = ld_state<Real, Store>(bin, vb0, ip * Nb, P, ip);
-        });
-      }
-    }
     // the forward needs the tile's right and top boundary faces too; the
     // adjoint-face pass leaves them to the next tile except at a wall
     const bool xhalo = active && t >= T::NROW && c == TX && r >= 1 && r <= TY;
@@ -484,7 +508,7 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
         Real fnew[Q];
         static_for<Q>([&](auto I) {
           constexpr int ip = decltype(I)::value;
-          Real v = fbc[ip] * Real(4.0 / 3.0) - fo[ip] * Real(1.0 / 3.0);
+          Real v = gc[ip];
           constexpr int kx = cross_index<0>(ip), ky = cross_index<1>(ip), kz = cross_index<2>(ip);
           if constexpr (kx >= 0) v -= Real(CX[ip]) * updx * (Cx[kx][ic(r - 1, c + 1)] - Cx[kx][ic(r - 1, c)]);
           if constexpr (ky >= 0) v -= Real(CY[ip]) * updy * (Dy[ky][id(r, c)] - Dy[ky][id(r - 1, c)]);
@@ -498,14 +522,8 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
         if (partial) {
           // bulk velocity of the new state for the flow-rate controller
           // (per-cell moments in the arithmetic type, the sum over cells in FP64)
-          Real rr = 0, jj[3] = {0, 0, 0};
-          static_for<Q>([&](auto I) {
-            constexpr int ip = decltype(I)::value;
-            rr += fnew[ip];
-            if constexpr (CX[ip] != 0) jj[0] += CX[ip] > 0 ? fnew[ip] : -fnew[ip];
-            if constexpr (CY[ip] != 0) jj[1] += CY[ip] > 0 ? fnew[ip] : -fnew[ip];
-            if constexpr (CZ[ip] != 0) jj[2] += CZ[ip] > 0 ? fnew[ip] : -fnew[ip];
-          });
+          Real rr, jj[3];
+          cell_moments<Real>(fnew, rr, jj);
           const Real js = as == 0 ? jj[0] : (as == 1 ? jj[1] : jj[2]);
           flow += (double)(js * rcp_real(rr));  // u along the flow axis; see k_flow_control
         }
@@ -519,6 +537,7 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
       static_for<Q>([&](auto I) {
         constexpr int ip = decltype(I)::value;
         fbc[ip] = fbn[ip];
+        if constexpr (!ADJ) gc[ip] = gn[ip];
         if constexpr (ADJ && ip != 0) lamc[ip] = lamn[ip];
       });
     }
@@ -527,18 +546,12 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
   }
 
   if (!ADJ && partial) {
-    // fixed-order block reduction of the flow sums
-    double* red = reinterpret_cast<double*>(&A[0][ia(0, 0)]);
-    // every column of the tile has the same z width (z is uniform here)
-    red[t] = interior ? flow * (P.wd[0][mi] * P.wd[1][mj] * P.wd[2][1]) : 0.0;
-    __syncthreads();
-    for (int s = 1; s < T::NT; s <<= 1) {
-      const int s2 = s << 1;
-      if ((t % s2) == 0 && t + s < T::NT) red[t] += red[t + s];
-      __syncthreads();
-    }
-    if (t == 0)
-      partial[tt] = red[0];
+    // fixed-order block reduction of the flow sums; every column of the tile
+    // has the same z width (z is uniform here)
+    double v[1] = {interior ? flow * (P.wd[0][mi] * P.wd[1][mj] * P.wd[2][1]) : 0.0};
+    constexpr int kSum[1] = {0};
+    block_reduce<1, T::NT>(v, kSum);
+    if (t == 0) partial[tt] = v[0];
   }
 }
 
@@ -546,17 +559,12 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64)
 // controller update (the law of k_flow_control).
 __global__ void __launch_bounds__(256) k_flow_finish(const double* partial, int nparts, double* famb, int axis,
                                                      double volume, double target, double gain, double dt) {
-  __shared__ double sh[256];
-  double s = 0;
-  for (int b = threadIdx.x; b < nparts; b += 256) s += partial[b];
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-    __syncthreads();
-  }
+  double s[1] = {0};
+  for (int b = threadIdx.x; b < nparts; b += 256) s[0] += partial[b];
+  constexpr int kSum[1] = {0};
+  block_reduce<1, 256>(s, kSum);
   if (threadIdx.x == 0) {
-    const double Ub = sh[0] / volume;
+    const double Ub = s[0] / volume;
     const double Fprev = famb[axis];
     double drag = Fprev;
     if (famb[4] != 0) drag = Fprev - (Ub - famb[3]) / dt;

@@ -97,29 +97,14 @@ __device__ __forceinline__ Real lerp_(Real a, Real b, Real m) {
 template <class Real>
 __device__ __forceinline__ void face_collide(Real (&ft)[Q], const Real F[3], Real hdt, Real fw0,
                                              Real fw1, Real fw2) {
-  Real rho = 0, jx = 0, jy = 0, jz = 0;
-  static_for<Q>([&](auto I) {
-    constexpr int ip = decltype(I)::value;
-    rho += ft[ip];
-    if constexpr (CX[ip] != 0) jx += CX[ip] > 0 ? ft[ip] : -ft[ip];
-    if constexpr (CY[ip] != 0) jy += CY[ip] > 0 ? ft[ip] : -ft[ip];
-    if constexpr (CZ[ip] != 0) jz += CZ[ip] > 0 ? ft[ip] : -ft[ip];
-  });
+  Real rho, j[3];
+  cell_moments<Real>(ft, rho, j);
   const Real inv = rcp_real(rho);
   const Real hh = Real(0.5) * hdt;
-  const Real ux = jx * inv + hh * F[0];
-  const Real uy = jy * inv + hh * F[1];
-  const Real uz = jz * inv + hh * F[2];
-  const Real uu = ux * ux + uy * uy + uz * uz;
-  const Real base = Real(1) - Real(1.5) * uu;
-  const Real uF = ux * F[0] + uy * F[1] + uz * F[2];
-  const Real P0 = fw2 - Real(3) * fw0 * uF;
+  const EqTerm<Real> eq(rho, j[0] * inv + hh * F[0], j[1] * inv + hh * F[1], j[2] * inv + hh * F[2], F, fw2, fw0);
   static_for<Q>([&](auto I) {
     constexpr int ip = decltype(I)::value;
-    const Real cu = Real(CX[ip]) * ux + Real(CY[ip]) * uy + Real(CZ[ip]) * uz;
-    const Real cF = Real(CX[ip]) * F[0] + Real(CY[ip]) * F[1] + Real(CZ[ip]) * F[2];
-    const Real feq = Real(wgt(ip)) * rho * (base + Real(3) * cu + Real(4.5) * cu * cu);
-    ft[ip] = fw1 * ft[ip] + (P0 + Real(3) * fw0 * cF) * feq;
+    ft[ip] = eq.template relax<ip>(fw1, ft[ip], cdot<ip>(F[0], F[1], F[2]));
   });
 }
 

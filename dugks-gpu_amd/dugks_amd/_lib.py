@@ -16,7 +16,6 @@ ERRORS = {
 }
 PREC_CONFIG, PREC_FP64, PREC_FP32, PREC_FP16S = -1, 0, 1, 2
 KERNEL_AUTO, KERNEL_TWOPASS, KERNEL_FUSED = -1, 0, 1
-ADJOINT_AUTO, ADJOINT_SPLIT, ADJOINT_ONEPASS = 0, 1, 2
 LOSS_WALL_SHEAR, LOSS_KINETIC_ENERGY = 0, 1
 
 
@@ -50,6 +49,7 @@ PROTOTYPES = {
     "dugks_sync": (_i, [_ctxp]),
     "dugks_time_steps": (_i, [_ctxp, _i, _i, _dp]),
     "dugks_get_dims": (_i, [_ctxp, _ip]),
+    "dugks_get_device": (_i, [_ctxp, _ip]),
     "dugks_get_params": (_i, [_ctxp, _dp]),
     "dugks_get_schedule": (_i, [_ctxp, _ip]),
     "dugks_write_outputs": (_i, [_ctxp, _i, _cp]),
@@ -69,10 +69,13 @@ PROTOTYPES = {
     "dugks_read_dump": (_i, [_ctxp, _i, _cp, _i]),
     "dugks_global_sums": (_i, [_ctxp, _dp]),
     "dugks_adjoint_step": (_i, [_ctxp, _vp, _vp, _vp]),
-    "dugks_set_adjoint_path": (_i, [_ctxp, _i]),
     "dugks_adjoint_step_f32": (_i, [_ctxp, _vp, _vp, _vp]),
     "dugks_loss": (_i, [_ctxp, _i, _dp, _vp]),
     "dugks_forward_adjoint": (_i, [_ctxp, _i, _i, _i, _i, _dp, _vp, _vp, _dp]),
+    "dugks_forward_record": (_i, [_ctxp, _i, _i, _i, _i, _dp]),
+    "dugks_adjoint_replay": (_i, [_ctxp, _vp, _vp, _dp]),
+    "dugks_release_sweep": (_i, [_ctxp]),
+    "dugks_sweep_bytes": (_i, [_ctxp, _dp]),
     # host-only helpers (no device needed)
     "dugks_config_derive": (_i, [_cp, _cp, _ip, _dp]),
     "dugks_config_spacing": (_i, [_cp, _cp, _i, _dp]),

@@ -1,12 +1,12 @@
 """torch.autograd.Function over the engine's hand-written adjoint.
 
-J(G) = (1/T) sum_{n=1..T} loss(f^n) where f^{n+1} = step(f^n; F_amb + G) and
-G is a per-cell centre body-force field (the FORCEX/Y/Z arrays of the
-reference's calc_fb1, src/dugks.cu:1131-1196).  The backward pass is the
-discrete adjoint with sqrt(T) checkpointing, run by dugks_forward_adjoint in
-the same call as the forward sweep (the checkpoints live in device memory
-only for the duration of that call), so forward() returns J and keeps dJ/dG
-for backward().
+J(G, f0) = (1/T) sum_{n=1..T} loss(f^n) where f^{n+1} = step(f^n; F_amb + G),
+f^0 = f0, and G is a per-cell centre body-force field (the FORCEX/Y/Z arrays
+of the reference's calc_fb1, src/dugks.cu:1131-1196).  forward() runs the T
+steps once, keeping sqrt(T) checkpoints on the engine (dugks_forward_record);
+backward() runs the discrete adjoint over them (dugks_adjoint_replay) and
+returns dJ/dG and dJ/df0.  Without a backward the checkpoints stay on the
+engine until its next sweep or release_sweep().
 """
 import torch
 
@@ -26,23 +26,23 @@ class ControlLoss(torch.autograd.Function):
         G = G.detach().contiguous()
         engine.set_f_device(f0.detach().contiguous())
         engine.set_cell_force_device(G)
-        if ctx.needs_input_grad[0]:
-            grad = torch.empty_like(G)
-            J, _ = engine.forward_adjoint(int(T), grad, loss_kind=loss_kind, interval=int(interval))
-            ctx.save_for_backward(grad)
-        else:
-            total = 0.0
-            for n in range(int(T)):
-                engine.step(1, istep0=1 + n)
-                total += engine.loss(loss_kind)
-            J = total / int(T)
-            ctx.save_for_backward(torch.zeros_like(G))
+        J = engine.forward_record(int(T), loss_kind=loss_kind, interval=int(interval))
+        ctx.engine = engine
+        ctx.save_for_backward(G)
         return torch.tensor(J, dtype=torch.float64, device=G.device)
 
     @staticmethod
     def backward(ctx, gJ):
-        (grad,) = ctx.saved_tensors
-        return gJ * grad, None, None, None, None, None
+        (G,) = ctx.saved_tensors
+        eng = ctx.engine
+        # the adjoint linearises about the recorded forward, whose control was G
+        eng.set_cell_force_device(G)
+        grad_G = torch.empty_like(G)
+        want_f0 = ctx.needs_input_grad[2]
+        nx, ny, nz = eng.shape
+        grad_f0 = torch.empty((19, nz, ny, nx), dtype=torch.float64, device=G.device) if want_f0 else None
+        eng.adjoint_replay(grad_G, grad_f0)
+        return gJ * grad_G, None, (gJ * grad_f0 if want_f0 else None), None, None, None
 
 
 def control_loss(G, engine, f0, T, loss_kind=_lib.LOSS_WALL_SHEAR, interval=0):

@@ -21,18 +21,30 @@ def _ptr(a):
     return a.ctypes.data_as(_dp)
 
 
-def _dev_ptr(t, dtype="float64"):
-    """Device pointer of a contiguous torch tensor of the given dtype (FP64
-    unless stated), or a raw int."""
-    if isinstance(t, int):
-        return ctypes.c_void_p(t)
+def _dev_ptr(t, numel, device, dtype="float64"):
+    """Device pointer of a torch tensor for a C entry point that reads or
+    writes `numel` values of `dtype` (FP64 unless stated) on HIP device
+    `device`.  The tensor must be a contiguous GPU tensor on that device with
+    exactly that many elements; anything else raises ValueError before the
+    pointer reaches the library.  A raw int is passed through unchecked (the
+    caller vouches for it); None stays None."""
     if t is None:
         return None
+    if isinstance(t, int):
+        return ctypes.c_void_p(t)
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError("device buffer must be a torch tensor or a raw pointer")
+    if not t.is_cuda:
+        raise ValueError("device buffer must be on the GPU")
+    if t.device.index != device:
+        raise ValueError(f"device buffer is on cuda:{t.device.index}, the engine on cuda:{device}")
+    if t.dtype != getattr(torch, dtype):
+        raise ValueError(f"device buffer must be {dtype}, got {t.dtype}")
     if not t.is_contiguous():
         raise ValueError("device buffer must be contiguous")
-    import torch
-    if t.dtype != getattr(torch, dtype):
-        raise ValueError(f"device buffer must be {dtype}")
+    if t.numel() != numel:
+        raise ValueError(f"device buffer holds {t.numel()} values, the entry point needs {numel}")
     return ctypes.c_void_p(t.data_ptr())
 
 
@@ -66,6 +78,9 @@ class Engine:
         if rc != _lib.OK:
             raise DugksError(rc, lib.dugks_init_error().decode())
         self._h = h
+        dev = ctypes.c_int()
+        lib.dugks_get_device(h, ctypes.byref(dev))
+        self.device = dev.value  # HIP device ordinal of the context
         dims = (ctypes.c_int * 3)()
         lib.dugks_get_dims(h, dims)
         self.shape = tuple(dims)  # (nx, ny, nz)
@@ -141,11 +156,16 @@ class Engine:
         f = np.ascontiguousarray(f, dtype=np.float64).reshape(self.fshape)
         self._check(self._lib.dugks_set_f(self._h, _ptr(f)))
 
+    def _dp(self, t, per_cell, dtype="float64"):
+        return _dev_ptr(t, per_cell * self.ncell, self.device, dtype)
+
     def get_f_device(self, dst):
-        self._check(self._lib.dugks_get_f_device(self._h, _dev_ptr(dst)))
+        """dst: float64 GPU tensor [19, nz, ny, nx] (SoA) on the engine's device."""
+        self._check(self._lib.dugks_get_f_device(self._h, self._dp(dst, 19)))
 
     def set_f_device(self, src):
-        self._check(self._lib.dugks_set_f_device(self._h, _dev_ptr(src)))
+        """src: float64 GPU tensor [19, nz, ny, nx] (SoA) on the engine's device."""
+        self._check(self._lib.dugks_set_f_device(self._h, self._dp(src, 19)))
 
     def macro(self):
         nx, ny, nz = self.shape
@@ -171,7 +191,7 @@ class Engine:
         self._check(self._lib.dugks_set_cell_force(self._h, *[_ptr(a) for a in arrs]))
 
     def set_cell_force_device(self, t):
-        self._check(self._lib.dugks_set_cell_force_device(self._h, _dev_ptr(t)))
+        self._check(self._lib.dugks_set_cell_force_device(self._h, self._dp(t, 3)))
 
     def global_sums(self):
         out = np.empty(8)
@@ -201,15 +221,8 @@ class Engine:
     # -- adjoint --
     def loss(self, kind=_lib.LOSS_WALL_SHEAR, grad=None):
         v = ctypes.c_double()
-        self._check(self._lib.dugks_loss(self._h, int(kind), ctypes.byref(v), _dev_ptr(grad)))
+        self._check(self._lib.dugks_loss(self._h, int(kind), ctypes.byref(v), self._dp(grad, 19)))
         return v.value
-
-    def set_adjoint_path(self, path):
-        """ADJOINT_AUTO or ADJOINT_SPLIT: the split path (adjoint-face pass +
-        per-cell gather); ADJOINT_ONEPASS: the one-pass adjoint kernel where
-        it applies (FP32 arithmetic, 16 x 12 tiles, at most one wall axis),
-        else the split path."""
-        self._check(self._lib.dugks_set_adjoint_path(self._h, int(path)))
 
     def adjoint_step(self, lam_next, lam_prev, grad_force=None):
         """lam_prev = (df^{n+1}/df^n)^T lam_next about the current state (and
@@ -218,7 +231,33 @@ class Engine:
         modes on the z-marching gather's configurations)."""
         lt = "float32" if str(getattr(lam_next, "dtype", "")).endswith("float32") else "float64"
         fn = self._lib.dugks_adjoint_step_f32 if lt == "float32" else self._lib.dugks_adjoint_step
-        self._check(fn(self._h, _dev_ptr(lam_next, lt), _dev_ptr(lam_prev, lt), _dev_ptr(grad_force)))
+        self._check(fn(self._h, self._dp(lam_next, 19, lt), self._dp(lam_prev, 19, lt),
+                       self._dp(grad_force, 3)))
+
+    def forward_record(self, T, loss_kind=_lib.LOSS_WALL_SHEAR, interval=0, istep0=1):
+        """T steps from the current state keeping sqrt(T) checkpoints on the
+        context; returns J = (1/T) sum_n loss(f^n)."""
+        J = ctypes.c_double()
+        self._check(self._lib.dugks_forward_record(self._h, int(istep0), int(T), int(loss_kind), int(interval),
+                                                   ctypes.byref(J)))
+        return J.value
+
+    def adjoint_replay(self, grad_force, grad_f0=None):
+        """Reverse sweep of the recorded forward: grad_force [3, nz, ny, nx]
+        gets dJ/dF_cell, grad_f0 [19, nz, ny, nx] (optional) dJ/df^0; returns
+        the device memory in use at the end of the sweep (bytes)."""
+        peak = ctypes.c_double()
+        self._check(self._lib.dugks_adjoint_replay(self._h, self._dp(grad_force, 3), self._dp(grad_f0, 19),
+                                                   ctypes.byref(peak)))
+        return peak.value
+
+    def release_sweep(self):
+        self._check(self._lib.dugks_release_sweep(self._h))
+
+    def sweep_bytes(self):
+        b = ctypes.c_double()
+        self._check(self._lib.dugks_sweep_bytes(self._h, ctypes.byref(b)))
+        return b.value
 
     def forward_adjoint(self, T, grad_force, loss_kind=_lib.LOSS_WALL_SHEAR, interval=0,
                         istep0=1, grad_f0=None):
@@ -226,7 +265,7 @@ class Engine:
         peak = ctypes.c_double()
         self._check(self._lib.dugks_forward_adjoint(self._h, int(istep0), int(T), int(loss_kind),
                                                     int(interval), ctypes.byref(J),
-                                                    _dev_ptr(grad_force), _dev_ptr(grad_f0),
+                                                    self._dp(grad_force, 3), self._dp(grad_f0, 19),
                                                     ctypes.byref(peak)))
         return J.value, peak.value
 

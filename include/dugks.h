@@ -95,6 +95,11 @@ int dugks_sync(dugks_ctx* ctx);
 /* Grid size nx, ny, nz (Numerics::nx/ny/nz). */
 int dugks_get_dims(const dugks_ctx* ctx, int dims[3]);
 
+/* HIP device ordinal the context runs on (the device chosen by init_cuda,
+ * src/cuda.cu:13-33; every device pointer passed to this context must be
+ * on it). */
+int dugks_get_device(const dugks_ctx* ctx, int* device);
+
 /* Derived parameters, in this order: dt, hdt, tau, visc, bw0, bw1, bw2, fw0,
  * fw1, fw2, dx, dy, dz (minimum width on a stretched axis), rho0,
  * precision, kernel (the test_params printout, src/dugks.cu:983-1016). */
@@ -130,7 +135,9 @@ int dugks_get_f(dugks_ctx* ctx, double* host_f);
 int dugks_set_f(dugks_ctx* ctx, const double* host_f);
 
 /* Device-side exchange in SoA layout f[ip][k][j][i], FP64, for callers that
- * keep state on the GPU (the torch wrapper). */
+ * keep state on the GPU (the torch wrapper).  Stream-ordered, no host sync.
+ * dugks_set_f_device restarts the flow-rate controller like dugks_set_f; it
+ * does not check the values for finiteness (dugks_set_f does). */
 int dugks_get_f_device(dugks_ctx* ctx, double* dev_f);
 int dugks_set_f_device(dugks_ctx* ctx, const double* dev_f);
 
@@ -183,17 +190,6 @@ int dugks_adjoint_step(dugks_ctx* ctx, const double* lam_next, double* lam_prev,
  * uses it internally where it applies. */
 int dugks_adjoint_step_f32(dugks_ctx* ctx, const float* lam_next, float* lam_prev, double* grad_force);
 
-/* Adjoint kernel path.  DUGKS_ADJOINT_AUTO (the default) and
- * DUGKS_ADJOINT_SPLIT run the split path (adjoint-face pass, then a per-cell
- * gather), the faster one on MI355X.  DUGKS_ADJOINT_ONEPASS runs the one-pass
- * kernel where it applies (FP32 arithmetic, 16 x 12 fused-step tiles, at most
- * one wall axis) and the split path elsewhere.  lam_prev must not overlap
- * lam_next on any path. */
-#define DUGKS_ADJOINT_AUTO 0
-#define DUGKS_ADJOINT_SPLIT 1
-#define DUGKS_ADJOINT_ONEPASS 2
-int dugks_set_adjoint_path(dugks_ctx* ctx, int path);
-
 /* Loss functionals J(f) = sum_cells g(f) evaluated on the current state and
  * their gradient dJ/df (device SoA FP64, written, not accumulated).
  * kind 0: mean wall shear stress (walls of the wall-normal axis, streamwise
@@ -203,15 +199,32 @@ int dugks_set_adjoint_path(dugks_ctx* ctx, int path);
 #define DUGKS_LOSS_KINETIC_ENERGY 1
 int dugks_loss(dugks_ctx* ctx, int kind, double* value, double* dev_grad);
 
-/* Forward T steps from the current state accumulating J = (1/T) sum_{n=1..T}
- * loss(f^n), then the reverse sweep with checkpoints every `interval` steps
- * held in device memory (interval <= 0: ceil(sqrt(T))), recomputing each
- * segment.  On return the state is f^T, *J holds the loss and dev_grad_force
- * ([3][nz][ny][nx] FP64) holds dJ/dF_cell; dev_grad_f0 (may be NULL) gets
- * dJ/df^0.  peak_bytes (may be NULL) gets the checkpoint memory used. */
+/* The sqrt(T)-checkpointed sweep in two calls.
+ *
+ * dugks_forward_record: T steps from the current state accumulating
+ * J = (1/T) sum_{n=1..T} loss(f^n) on the device, keeping f every `interval`
+ * steps (interval <= 0: ceil(sqrt(T)); clamped to T) and f^T in a checkpoint
+ * pool on the context.  On return the state is f^T and *J holds the loss.
+ *
+ * dugks_adjoint_replay: the reverse sweep of the recorded forward, each
+ * segment recomputed from its checkpoint.  dev_grad_force ([3][nz][ny][nx]
+ * FP64) gets dJ/dF_cell, dev_grad_f0 ([19][nz][ny][nx] FP64, may be NULL)
+ * gets dJ/df^0.  It uses the context's cell force, which must be the one the
+ * forward was recorded with; the context's state is not used or changed, so
+ * other work may run between the two calls.  peak_bytes (may be NULL) gets
+ * the device memory in use (hipMemGetInfo) at the end of the sweep, when all
+ * of its buffers are allocated.  The pool stays allocated for the next
+ * sweep until dugks_release_sweep or dugks_free.
+ *
+ * dugks_forward_adjoint is the two in one call. */
+int dugks_forward_record(dugks_ctx* ctx, int istep0, int T, int loss_kind, int interval, double* J);
+int dugks_adjoint_replay(dugks_ctx* ctx, double* dev_grad_force, double* dev_grad_f0, double* peak_bytes);
 int dugks_forward_adjoint(dugks_ctx* ctx, int istep0, int T, int loss_kind, int interval,
                           double* J, double* dev_grad_force, double* dev_grad_f0,
                           double* peak_bytes);
+/* Frees the checkpoint pool; *bytes of dugks_sweep_bytes is its current size. */
+int dugks_release_sweep(dugks_ctx* ctx);
+int dugks_sweep_bytes(dugks_ctx* ctx, double* bytes);
 
 /* ---- Host-only helpers: no device is touched; messages via dugks_init_error ---- */
 

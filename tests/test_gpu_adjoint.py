@@ -98,70 +98,6 @@ def test_adjoint_step_matches_oracle(rng, case, kernel, precision, tol):
     assert np.abs(g - g_ref).max() <= tol * np.abs(g_ref).max()
 
 
-# configurations of the one-pass adjoint kernel (FP32 arithmetic, 16 x 12
-# tiles): several tiles per axis, interior tiles away from the walls, and
-# several z-chunks (their boundary planes are recomputed by both chunks)
-ONEPASS_CASES = {
-    "channel": dict(nx=48, ny=36, nz=40, periods_y=0, mesh_y="tanh:0.85", dt_rule="min", ambient_fx=2e-4,
-                    xwidth=4.8, ywidth=2.0, zwidth=4.0),
-    "walls_x": dict(nx=48, ny=24, nz=20, periods_x=0, mesh_x="tanh:0.8", dt_rule="min", ambient_fz=2e-4,
-                    xwidth=4.8, ywidth=2.4, zwidth=2.0),
-    "periodic": dict(nx=32, ny=24, nz=8, xwidth=3.2, ywidth=2.4, zwidth=0.8, ambient_fx=1e-4),
-}
-
-
-@pytest.mark.parametrize("case", list(ONEPASS_CASES))
-@pytest.mark.parametrize("precision", [PREC_FP32, PREC_FP16S], ids=["fp32", "fp16"])
-def test_onepass_adjoint_matches_split_path(rng, case, precision):
-    text = configs.base(visc=0.02, **ONEPASS_CASES[case])
-    eng = _engine(text, precision=precision)
-    eng.set_f(random_state(eng.shape, rng))
-    nx, ny, nz = eng.shape
-    eng.set_cell_force(*(1e-4 * rng.standard_normal((3, nz, ny, nx))))
-    dev = torch.device("cuda")
-    lam = torch.from_numpy(rng.standard_normal((19, nz, ny, nx))).to(dev)
-    out = {}
-    for path in (dugks_amd.ADJOINT_SPLIT, dugks_amd.ADJOINT_ONEPASS):
-        eng.set_adjoint_path(path)
-        lp = torch.empty_like(lam)
-        g = torch.full((3, nz, ny, nx), 0.5, dtype=torch.float64, device=dev)  # the gradient is added to
-        eng.adjoint_step(lam, lp, g)
-        torch.cuda.synchronize()
-        out[path] = (lp.cpu().numpy(), g.cpu().numpy() - 0.5)
-    (l0, g0), (l1, g1) = out[dugks_amd.ADJOINT_SPLIT], out[dugks_amd.ADJOINT_ONEPASS]
-    # the one-pass kernel ran: its FP32 sums run in another order
-    assert not np.array_equal(l0, l1)
-    assert np.abs(l1 - l0).max() <= 2e-5 * np.abs(l0).max()
-    assert np.abs(g1 - g0).max() <= 2e-5 * np.abs(g0).max()
-
-
-@pytest.mark.parametrize("case", ["tall", "tall_walls_x", "tall_periodic"])
-def test_onepass_adjoint_matches_oracle(rng, case):
-    text = configs.base(visc=0.02, **CASES[case])
-    eng = _engine(text, precision=PREC_FP32)
-    eng.set_adjoint_path(dugks_amd.ADJOINT_ONEPASS)
-    ora, famb = _oracle(eng, text)
-    eng.set_f(random_state(eng.shape, rng))
-    f0 = eng.get_f()
-    nx, ny, nz = eng.shape
-    G = 1e-4 * rng.standard_normal((3, nz, ny, nx))
-    eng.set_cell_force(*G)
-    ora.set_f(f0)
-    ora.set_cell_force(*(G + famb.reshape(3, 1, 1, 1)))
-    lam = rng.standard_normal(f0.shape)
-    lp_ref, g_ref = ora.adjoint_step(lam)
-    dev = torch.device("cuda")
-    lam_d = torch.from_numpy(_soa(lam)).to(dev)
-    lp_d = torch.empty_like(lam_d)
-    g_d = torch.zeros((3, nz, ny, nx), dtype=torch.float64, device=dev)
-    eng.adjoint_step(lam_d, lp_d, g_d)
-    torch.cuda.synchronize()
-    lp = _aos(lp_d.cpu().numpy())
-    g = np.moveaxis(g_d.cpu().numpy(), 0, -1)
-    assert np.abs(lp - lp_ref).max() <= 2e-5 * np.abs(lp_ref).max()
-    assert np.abs(g - g_ref).max() <= 2e-5 * np.abs(g_ref).max()
-
-
 # configurations of the z-marching gather (nx % 32 == 0, ny % 8 == 0)
 LAM32_CASES = {
     "channel": dict(nx=64, ny=24, nz=40, periods_y=0, mesh_y="tanh:0.85", dt_rule="min", ambient_fx=2e-4,
@@ -198,8 +134,10 @@ def test_adjoint_step_f32_lambda_matches_f64_lambda(rng, case, precision):
 
 def test_adjoint_step_f32_lambda_unsupported_configurations():
     dev = torch.device("cuda")
+    # FP64 arithmetic, and a grid the z-marching gather does not cover (nx % 32 != 0)
+    wide = dict(nx=48, ny=36, nz=8, periods_y=0, ambient_fx=2e-4, xwidth=4.8, ywidth=2.0, zwidth=0.8)
     for text, prec in ((configs.base(visc=0.02, **LAM32_CASES["periodic"]), PREC_FP64),
-                       (configs.base(visc=0.02, **ONEPASS_CASES["channel"]), PREC_FP32)):
+                       (configs.base(visc=0.02, **wide), PREC_FP32)):
         eng = _engine(text, precision=prec)
         nx, ny, nz = eng.shape
         lam = torch.zeros((19, nz, ny, nx), dtype=torch.float32, device=dev)
@@ -207,31 +145,8 @@ def test_adjoint_step_f32_lambda_unsupported_configurations():
             eng.adjoint_step(lam, torch.empty_like(lam))
 
 
-def test_sweep_with_f32_lambda_matches_f64_lambda(rng):
-    # dugks_forward_adjoint keeps lam in FP32 where it can; the one-pass path
-    # selector falls back to the split path with FP64 lam on this grid (ny %
-    # 12 != 0), which must give the same bits
-    eng, f0, G, T = _adjoint_problem(rng, PREC_FP32, nx=32, ny=16, nz=8, T=12)
-    nx, ny, nz = eng.shape
-    dev = torch.device("cuda")
-    res = []
-    for path in (dugks_amd.ADJOINT_AUTO, dugks_amd.ADJOINT_ONEPASS):
-        eng.set_adjoint_path(path)
-        eng.set_f(f0)
-        eng.set_cell_force(*G)
-        gd = torch.empty((3, nz, ny, nx), dtype=torch.float64, device=dev)
-        g0 = torch.empty((19, nz, ny, nx), dtype=torch.float64, device=dev)
-        J, peak = eng.forward_adjoint(T, gd, interval=4, grad_f0=g0)
-        torch.cuda.synchronize()
-        res.append((J, gd.cpu().numpy(), g0.cpu().numpy(), peak))
-    assert res[0][0] == res[1][0]
-    assert np.array_equal(res[0][1], res[1][1]) and np.abs(res[0][1]).max() > 0
-    assert np.array_equal(res[0][2], res[1][2])
-    assert res[0][3] < res[1][3]  # FP32 lam buffers
-
-
 def test_adjoint_step_rejects_overlapping_buffers(rng):
-    eng = _engine(configs.base(visc=0.02, **ONEPASS_CASES["periodic"]), precision=PREC_FP32)
+    eng = _engine(configs.base(visc=0.02, **LAM32_CASES["periodic"]), precision=PREC_FP32)
     nx, ny, nz = eng.shape
     lam = torch.zeros((19, nz, ny, nx), dtype=torch.float64, device=torch.device("cuda"))
     with pytest.raises(dugks_amd.DugksError):
@@ -324,13 +239,12 @@ def test_forward_adjoint_matches_central_differences(rng):
         assert abs(fd - an) <= 1e-6 * abs(an), (trial, fd, an)
 
 
-@pytest.mark.parametrize("ny,path", [(16, 0), (24, 2)], ids=["split", "onepass"])
-def test_fp32_gradient_against_fp64(rng, ny, path):
-    # ny = 24 gives 16 x 12 tiles, where the one-pass adjoint applies
+@pytest.mark.parametrize("ny", [16, 24])
+def test_fp32_gradient_against_fp64(rng, ny):
+    # ny = 16: the z-marching gather with FP32 lam; ny = 24: 16 x 12 step tiles
     e64, f64, G, T = _adjoint_problem(rng, PREC_FP64, ny=ny)
     J64, g64, _ = _grad(e64, f64, G, T)
     e32, f32, _, _ = _adjoint_problem(rng, PREC_FP32, ny=ny)
-    e32.set_adjoint_path(path)
     J32, g32, _ = _grad(e32, f32, G, T)
     assert abs(J32 - J64) <= 1e-5 * abs(J64)
     assert np.abs(g32 - g64).max() <= 1e-3 * np.abs(g64).max()
@@ -380,16 +294,41 @@ def test_autograd_function(rng):
     f0 = torch.empty((19, nz, ny, nx), dtype=torch.float64, device=dev)
     eng.get_f_device(f0)
     G = (1e-6 * torch.randn((3, nz, ny, nx), dtype=torch.float64, device=dev)).requires_grad_(True)
-    J = control_loss(G, eng, f0, 8)
+    f0g = f0.clone().requires_grad_(True)
+    eng.release_sweep()
+    J = control_loss(G, eng, f0g, 8)
+    # forward() ran only the forward sweep: checkpoints, no adjoint buffers yet
+    fwd_bytes = eng.sweep_bytes()
+    assert fwd_bytes > 0
     J.backward()
-    g1 = G.grad.clone()
+    assert eng.sweep_bytes() > fwd_bytes
+    g1, gf1 = G.grad.clone(), f0g.grad.clone()
     gd = torch.empty_like(g1)
+    g0 = torch.empty_like(f0)
     eng.set_f_device(f0)
     eng.set_cell_force_device(G.detach())
-    J2, _ = eng.forward_adjoint(8, gd)
+    J2, _ = eng.forward_adjoint(8, gd, grad_f0=g0)
     torch.cuda.synchronize()
     assert abs(J.item() - J2) <= 1e-14 * abs(J2)
     assert torch.equal(g1, gd)
+    assert torch.equal(gf1, g0) and float(g0.abs().max()) > 0
     with torch.no_grad():
         J3 = control_loss(G.detach(), eng, f0, 8)
     assert abs(J3.item() - J2) <= 1e-13 * abs(J2)
+
+
+def test_device_pointers_are_checked():
+    eng = _engine(configs.base(visc=0.02, **LAM32_CASES["periodic"]), precision=PREC_FP32)
+    nx, ny, nz = eng.shape
+    dev = torch.device("cuda", eng.device)
+    ok = torch.zeros((19, nz, ny, nx), dtype=torch.float64, device=dev)
+    eng.set_f_device(ok)
+    bad = [torch.zeros((19, nz, ny, nx - 1), dtype=torch.float64, device=dev),  # size
+           torch.zeros((19, nz, ny, nx), dtype=torch.float32, device=dev),      # dtype
+           torch.zeros((19, nx, ny, nz), dtype=torch.float64, device=dev).transpose(1, 3),  # layout
+           torch.zeros((19, nz, ny, nx), dtype=torch.float64)]                  # host memory
+    for t in bad:
+        with pytest.raises(ValueError):
+            eng.set_f_device(t)
+    with pytest.raises(ValueError):
+        eng.set_cell_force_device(ok)  # needs 3 values per cell

@@ -1,0 +1,157 @@
+"""The benchmark workloads at full size on the GPU.
+
+- Taylor-Green vortex, 128^3 periodic, u0 = 0.05, Re = 1600, 1000 steps in
+  FP64 and FP32: mass conserved, kinetic energy decays monotonically, and
+  the FP32 kinetic energy and dissipation rate agree with FP64.
+- Re_tau = 180 channel, 128^3, tanh-stretched wall-normal mesh, constant
+  flow rate, 10 000 steps: mean-velocity, rms and Reynolds-stress profiles
+  of each reduced-precision storage mode against FP64, and the bulk velocity
+  held by the controller.
+- Large channel, 256 x 192 x 256, 200 steps per mode: finite, mass conserved,
+  bulk velocity held, FP32 against FP64.
+
+Field errors are relative to the FP64 maxima (each profile to its own
+maximum), the definitions bench.py reports.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import dugks_amd
+from dugks_amd import PREC_FP16S, PREC_FP32, PREC_FP64, configs
+
+pytestmark = pytest.mark.gpu
+
+# field-error bar of the reduced-precision modes against FP64
+FIELD_TOL = 1e-3
+
+
+def _engine(text, precision):
+    return dugks_amd.Engine(text, precision=precision, device=0)
+
+
+def _velocity_stats(eng):
+    """u, v, w fields and the wall-normal (y) profiles of mean u, the rms
+    velocities and <u'v'>, FP64 on the host."""
+    rho, ux, uy, uz = eng.macro()  # [k][j][i]
+    mean = lambda a: a.mean(axis=(0, 2))
+    U, V, Wm = mean(ux), mean(uy), mean(uz)
+    prof = {"U": U,
+            "urms": np.sqrt(np.maximum(mean(ux * ux) - U * U, 0)),
+            "vrms": np.sqrt(np.maximum(mean(uy * uy) - V * V, 0)),
+            "wrms": np.sqrt(np.maximum(mean(uz * uz) - Wm * Wm, 0)),
+            "uv": mean(ux * uy) - U * V}
+    return {"rho": rho, "u": (ux, uy, uz), "prof": prof}
+
+
+def _field_errors(ref, got):
+    du2 = sum((g - r) ** 2 for g, r in zip(got["u"], ref["u"]))
+    umax = float(np.sqrt(sum(r * r for r in ref["u"]).max()))
+    out = {"rho_max": float(np.abs(got["rho"] - ref["rho"]).max() / np.abs(ref["rho"]).max()),
+           "u_max": float(np.sqrt(du2.max()) / umax)}
+    for k, r in ref["prof"].items():
+        out[k] = float(np.abs(got["prof"][k] - r).max() / np.abs(r).max())
+    return out
+
+
+def _bulk(eng):
+    """Momentum bulk velocity <j/rho> along x (the controlled quantity)."""
+    s = eng.global_sums()
+    return s[4] / s[7] - eng.params["hdt"] * eng.ambient_force()[0]
+
+
+def _dissipation(ux, uy, uz, L, nu):
+    """eps = 2 nu <S:S> of a periodic cube field [k][j][i] (central differences)."""
+    n = ux.shape[0]
+    h = L / n
+    d = lambda a, ax: (np.roll(a, -1, axis=ax) - np.roll(a, 1, axis=ax)) / (2 * h)
+    g = [[d(c, 2 - b) for b in range(3)] for c in (ux, uy, uz)]  # g[a][b] = du_a/dx_b
+    ss = sum((0.5 * (g[a][b] + g[b][a])) ** 2 for a in range(3) for b in range(3))
+    return 2 * nu * float(ss.mean())
+
+
+def test_taylor_green_128_fp32_against_fp64():
+    steps, every = 1000, 20
+    E, mass, eps = {}, {}, {}
+    for name, prec in (("fp64", PREC_FP64), ("fp32", PREC_FP32)):
+        text = configs.taylor_green(n=128, precision=name, nsteps=steps)
+        eng = _engine(text, prec)
+        m0 = eng.global_sums()[0]
+        series = [eng.loss(dugks_amd.LOSS_KINETIC_ENERGY)]
+        for k in range(steps // every):
+            eng.step(every, istep0=1 + k * every)
+            series.append(eng.loss(dugks_amd.LOSS_KINETIC_ENERGY))
+        mass[name] = abs(eng.global_sums()[0] - m0) / m0
+        E[name] = np.array(series)
+        _, ux, uy, uz = eng.macro()
+        eps[name] = _dissipation(ux, uy, uz, 2 * math.pi, eng.params["visc"])
+        eng.close()
+    assert mass["fp64"] <= 1e-12
+    assert mass["fp32"] <= 1e-5
+    for name in E:
+        assert np.all(np.diff(E[name]) < 0), name  # monotone decay
+    # the vortex has decayed measurably over the run
+    assert E["fp64"][-1] < 0.99 * E["fp64"][0]
+    assert np.abs(E["fp32"] - E["fp64"]).max() / np.abs(E["fp64"]).max() <= FIELD_TOL
+    assert abs(eps["fp32"] - eps["fp64"]) / eps["fp64"] <= FIELD_TOL
+
+
+@pytest.fixture(scope="module")
+def channel_128():
+    """FP64 reference run of the 128^3 Re_tau = 180 channel, 10 000 steps."""
+    steps = 10000
+    eng = _engine(configs.turbulent_channel(precision="fp64", nsteps=steps), PREC_FP64)
+    ub0 = _bulk(eng)
+    eng.step(steps, istep0=1)
+    stats = _velocity_stats(eng)
+    ub = _bulk(eng)
+    eng.close()
+    return steps, stats, ub0, ub
+
+
+# the deviation codec stores f/w - rho0 in FP16, which rounds the 3 c.u part
+# of every population each step: its error grows past the bar over 10 000
+# steps of this flow
+_FP16S_MISSES = pytest.mark.xfail(strict=True, reason="FP16 deviation storage rounds 3 c.u each step; "
+                                                      "known to exceed the 1e-3 field-error bar here")
+
+
+@pytest.mark.parametrize("mode", ["fp32", pytest.param("fp16", marks=_FP16S_MISSES)])
+def test_channel_128_storage_modes_against_fp64(channel_128, mode):
+    steps, ref, ub0_ref, ub_ref = channel_128
+    prec = {"fp32": PREC_FP32, "fp16": PREC_FP16S}[mode]
+    eng = _engine(configs.turbulent_channel(precision=mode, nsteps=steps), prec)
+    ub0 = _bulk(eng)
+    eng.step(steps, istep0=1)
+    got = _velocity_stats(eng)
+    ub = _bulk(eng)
+    eng.close()
+    # the controller holds the bulk velocity in both runs
+    assert abs(ub_ref - ub0_ref) <= 1e-6 * ub0_ref
+    assert abs(ub - ub0) <= 1e-5 * ub0
+    err = _field_errors(ref, got)
+    bad = {k: v for k, v in err.items() if not v <= FIELD_TOL}
+    assert not bad, err
+
+
+def test_large_channel_256x192x256():
+    steps = 200
+    out = {}
+    for name, prec in (("fp64", PREC_FP64), ("fp32", PREC_FP32), ("fp16", PREC_FP16S)):
+        eng = _engine(configs.large_channel(precision=name, nsteps=steps), prec)
+        s0 = eng.global_sums()
+        ub0 = _bulk(eng)
+        eng.step(steps, istep0=1)
+        s1 = eng.global_sums()
+        assert all(math.isfinite(v) for v in s1), name
+        # mass: round-off of the arithmetic type (FP16 storage rounds each state)
+        tol = {"fp64": 1e-12, "fp32": 1e-5, "fp16": 1e-3}[name]
+        assert abs(s1[0] - s0[0]) / s0[0] <= tol, name
+        assert abs(_bulk(eng) - ub0) <= 1e-5 * ub0, name
+        if name != "fp16":
+            out[name] = _velocity_stats(eng)
+        eng.close()
+    err = _field_errors(out["fp64"], out["fp32"])
+    bad = {k: v for k, v in err.items() if not v <= FIELD_TOL}
+    assert not bad, err

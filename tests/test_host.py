@@ -202,29 +202,55 @@ def test_driver_reports_bad_config(tmp_path):
     assert "cannot open config file" in r.stderr
 
 
-def test_velmean_postprocessing(tmp_path):
-    # ProcessProfiles.py (examples/channel/ProcessProfiles.py:17-79): blocks
-    # up to ignore_before are skipped, the rest averaged, the halves merged
-    # (mean and variances even, shear stress odd about the channel centre)
-    n = 6
-    rng = np.random.default_rng(5)
-    blocks = {step: rng.standard_normal((n, 9)) for step in (100, 200, 300, 400)}
-    path = tmp_path / "profiles-2.dat"
-    with open(path, "w") as f:
-        for step, b in blocks.items():
-            f.write(f"{step:08d}\n")
-            for row in b:
-                f.write(" ".join(f"{x: .17e}" for x in row) + "\n")
+def test_velmean_matches_reference_script_output(tmp_path):
+    # golden pair written by tests/golden/make_velmean_golden.py: the
+    # reference's examples/channel/ProcessProfiles.py run on a synthetic
+    # profiles-2.dat (8 rows, blocks at steps 1, 5001, ..., 20001, blocks
+    # after step 5001 averaged, mean = column 2 (w), stress = column 3 (<u'w'>)).
+    # The C++ restatement must write the same bytes.
+    src = os.path.join(GOLDEN, "velmean_profiles-2.dat")
     out = tmp_path / "VELMEAN.dat"
-    dugks_amd.velmean(str(path), n, str(out), ignore_before=150, mean_col=0, stress_col=4)
-    got = np.loadtxt(out)
-    avg = np.mean([blocks[s] for s in (200, 300, 400)], axis=0)
-    assert np.array_equal(got[:, 0], np.arange(1, n + 1))
-    assert np.allclose(got[:, 1], 0.5 * (avg[:, 0] + avg[::-1, 0]), rtol=1e-14, atol=1e-15)
-    assert np.allclose(got[:, 2], 0.5 * (avg[:, 4] - avg[::-1, 4]), rtol=1e-14, atol=1e-15)
-    for q in range(3):
-        assert np.allclose(got[:, 3 + q], 0.5 * (avg[:, 6 + q] + avg[::-1, 6 + q]), rtol=1e-14, atol=1e-15)
-    # symmetry of the merged profiles
-    assert np.allclose(got[:, 1], got[::-1, 1]) and np.allclose(got[:, 2], -got[::-1, 2])
+    dugks_amd.velmean(src, 8, str(out), ignore_before=5001, mean_col=2, stress_col=3)
+    with open(os.path.join(GOLDEN, "velmean_VELMEAN.dat"), "rb") as f:
+        expect = f.read()
+    assert out.read_bytes() == expect
     with pytest.raises(dugks_amd.DugksError):
-        dugks_amd.velmean(str(path), n, str(out), ignore_before=1000)
+        dugks_amd.velmean(src, 8, str(out), ignore_before=10 ** 6)
+
+
+def test_velmean_writes_python_repr(tmp_path):
+    # VELMEAN.dat rows are Python f-strings of floats: every value is written
+    # as repr() writes it.  A one-block, two-row file with mean (x, x) and
+    # stress (x, -x) makes the first output row hold x itself.
+    rng = np.random.default_rng(11)
+    vals = [0.0, -0.0, 1.0, -2.5, 0.1, 1e-4, 9.999999999999999e-05, 1e-5, 1.5e-7, 123.456,
+            1e15, 9999999999999998.0, 1e16, 1.2345678901234567e17, 5e-324, 1.7976931348623157e308,
+            2.0 ** 60, 1 / 3, 2 / 3, 4.8080371999999990e-05]
+    vals += list(rng.standard_normal(40) * 10.0 ** rng.integers(-12, 18, 40))
+    for k, x in enumerate(vals):
+        src = tmp_path / f"p{k}.dat"
+        row = [x, 0, 0, x, 0, 0, x, x, x]
+        with open(src, "w") as f:
+            f.write("00000001\n")
+            f.write(" ".join(repr(float(v)) for v in row) + "\n")
+            f.write(" ".join(repr(float(v)) for v in [x, 0, 0, -x, 0, 0, x, x, x]) + "\n")
+        out = tmp_path / f"v{k}.dat"
+        dugks_amd.velmean(str(src), 2, str(out), ignore_before=0, mean_col=0, stress_col=3)
+        first = out.read_text().splitlines()[0].split()
+        a = 0.0 + float(x)  # the block sums start from 0.0, as the script's zeros arrays do
+        b = 0.0 - float(x)
+        assert first == ["1.0", repr(0.5 * (a + a)), repr(0.5 * (a - b)), repr(0.5 * (a + a)),
+                         repr(0.5 * (a + a)), repr(0.5 * (a + a))], (x, first)
+
+
+def test_device_pointer_checks_reject_host_tensors():
+    # a host tensor or a foreign object never reaches a C entry point as a
+    # device pointer; None and raw integers pass through
+    torch = pytest.importorskip("torch")
+    from dugks_amd.engine import _dev_ptr
+    with pytest.raises(ValueError):
+        _dev_ptr(torch.zeros(19 * 8, dtype=torch.float64), 19 * 8, 0)
+    with pytest.raises(ValueError):
+        _dev_ptr(np.zeros(19 * 8), 19 * 8, 0)
+    assert _dev_ptr(None, 19, 0) is None
+    assert _dev_ptr(1234, 19, 0).value == 1234

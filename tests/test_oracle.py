@@ -229,10 +229,12 @@ def test_mirror_equivariance(rng, case):
     assert np.abs(reflect_state(a.get_f(), axis) - b.get_f()).max() < 1e-14
 
 
-@pytest.mark.parametrize("stretched", [False, True])
-def test_reconstruction_exact_for_linear_fields(rng, stretched):
+@pytest.mark.parametrize("stretched,recon", [(False, "tri"), (True, "tri"), (False, "grad")])
+def test_reconstruction_exact_for_linear_fields(rng, stretched, recon):
     # with walls on every axis the ghost extrapolation is linear, so a linear
-    # f-bar field is reconstructed exactly at the foot point x_face - c h
+    # f-bar field is reconstructed exactly at the foot point x_face - c h, by
+    # the trilinear scheme (calc_fx_tri) and by the midpoint + gradient scheme
+    # (calc_fx, uniform axes), for all 19 populations
     shape = (5, 6, 4)
     width = (1.0, 1.3, 0.8)
     dt = 0.02
@@ -241,7 +243,7 @@ def test_reconstruction_exact_for_linear_fields(rng, stretched):
     if stretched:
         w = np.array([0.05, 0.12, 0.3, 0.35, 0.31, 0.17])
         spacing[1] = w / w.sum() * width[1]
-    o = Oracle(shape, (0, 0, 0), width, dt=dt, visc=0.01, spacing=spacing)
+    o = Oracle(shape, (0, 0, 0), width, dt=dt, visc=0.01, spacing=spacing, recon=recon)
     faces_x = []
     for a in range(3):
         d = spacing[a] if spacing[a] is not None else np.full(shape[a], width[a] / shape[a])
@@ -269,6 +271,42 @@ def test_reconstruction_exact_for_linear_fields(rng, stretched):
                     if not stretched:
                         expect0 = A[0] + xf @ B[0]
                         assert abs(faces[k, j, i, s, 0] - expect0) < 1e-12
+
+
+def test_gradient_minus_trilinear_on_quadratic_fields(rng):
+    # f-bar = sum_b K_b x_b^2 (per population).  Along the face normal both
+    # schemes make the same error (the normal lerp / midpoint-slope pair is
+    # exact to the same quadratic term); along a tangential axis b the
+    # trilinear lerp between the cell row and the upwind row has error
+    # h d_b |c_b| - h^2 c_b^2 at the foot point and the central-difference
+    # gradient has error -h^2 c_b^2, so tri - grad = h d_b |c_b| summed over
+    # the tangential axes.  A wrong sign, a one-sided difference or d instead
+    # of 2 d in a tangential gradient of calc_fx breaks this.
+    shape = (6, 7, 5)
+    width = (1.2, 1.4, 0.5)
+    d = [w / n for w, n in zip(width, shape)]
+    dt = 0.03
+    h = dt / 2
+    centres = [(np.arange(n) + 0.5) * dd for n, dd in zip(shape, d)]
+    X, Y, Z = np.meshgrid(*centres, indexing="ij")
+    pos = np.stack([X, Y, Z], -1).transpose(2, 1, 0, 3)  # (z,y,x,3)
+    K = rng.standard_normal((19, 3))
+    fbar = 2.0 + (pos ** 2) @ K.T
+    faces = {}
+    for recon in ("tri", "grad"):
+        o = Oracle(shape, (0, 0, 0), width, dt=dt, visc=0.01, recon=recon)
+        faces[recon] = o.reconstruct(fbar)
+    diff = faces["tri"] - faces["grad"]
+    nx, ny, nz = shape
+    # cells whose stencils (one row beyond each tangential neighbour) stay
+    # away from the wall ghosts, whose extrapolation is not exact here
+    sl = (slice(1, nz - 1), slice(1, ny - 1), slice(1, nx - 1))
+    for s in range(6):
+        a = s // 2
+        expect = sum(K[:, b] * h * d[b] * np.abs(C[:, b]) for b in range(3) if b != a)
+        got = diff[sl][..., s, :]
+        assert np.abs(got - expect).max() < 1e-13, s
+    assert np.abs(diff).max() > 1e-4
 
 
 def test_gradient_and_trilinear_agree_on_normal_populations(rng):
@@ -382,3 +420,60 @@ def test_perturb_force_structure():
         assert np.abs(f1[a][:, :, band]).max() > 0
         assert np.allclose(f2[a][:, :, band], r * f1[a][:, :, band], rtol=1e-12, atol=1e-20)
     assert np.allclose(f2[2][:, :, band] - FF, r * (f1[2][:, :, band] - FF), rtol=1e-12, atol=1e-12 * FF)
+
+
+def test_perturb_force_spatial_pattern():
+    # calc_perturb_force (src/dugks.cu:1086-1126) in the band: with
+    # X = 2 pi (i - 3) dx / (Lx / 4), Y = 2 pi (j - 1/2) dy / Ly,
+    # Z = 2 pi (k - 1/2) dz / Lz (1-based cell indices),
+    #   F'x ~ (1 - cos X) cos(3 Y) cos(2 Z),
+    #   F'y ~ sin X cos(3 Y) sin(2 Z),
+    #   F'z - FF ~ sin X sin(3 Y) cos(2 Z).
+    # Projected on the Fourier modes of the cell-centred y and z grids each
+    # component holds one mode, beta = 3 in y and gamma = 2 in z, with the
+    # stated cos / sin phase (a real or an imaginary coefficient); along x it
+    # follows its band profile, even or odd about X = pi.
+    from pyoracle import perturb_force
+    nx, ny, nz = 64, 24, 16
+    width = (6.4, 2.4, 1.6)
+    d = tuple(w / n for w, n in zip(width, (nx, ny, nz)))
+    dt, visc, step = 0.5, 0.02, 250
+    f = perturb_force((nx, ny, nz), width, d, dt, visc, step, (1e-4, 0.0, 0.0))
+    ustar = 2 * 180 * visc / width[0]
+    FF = 2 * ustar ** 2 / width[0]
+    band = np.arange(3, nx // 4 + 1)  # 0-based i of 1-based 4 .. nx/4 + 1
+    comps = [f[0][:, :, band], f[1][:, :, band], f[2][:, :, band] - FF]  # [k][j][m]
+    jj, kk = np.arange(ny) + 0.5, np.arange(nz) + 0.5
+    ey = np.exp(-2j * np.pi * np.outer(np.arange(ny), jj) / ny)  # [mode][j]
+    ez = np.exp(-2j * np.pi * np.outer(np.arange(nz), kk) / nz)
+    # (y phase, z phase) of each component: 0 = cos (real), 1 = sin (imaginary)
+    phases = [(0, 0), (0, 1), (1, 0)]
+    X = 2 * np.pi * (band + 1 - 3) / (nx // 4)
+    shapes = [1 - np.cos(X), np.sin(X), np.sin(X)]
+    for comp, (py, pz), sx in zip(comps, phases, shapes):
+        c = np.einsum("my,nz,zyx->nmx", ey, ez, comp)  # [z mode][y mode][x]
+        scale = np.abs(c).max()
+        assert scale > 0
+        modes = np.zeros(c.shape[:2], bool)
+        modes[np.ix_([2, nz - 2], [3, ny - 3])] = True
+        assert np.abs(c[~modes]).max() <= 1e-12 * scale
+        c32 = c[2, 3]
+        real, imag = np.abs(c32.real).max(), np.abs(c32.imag).max()
+        if py + pz == 1:   # one sine factor: imaginary coefficient
+            assert real <= 1e-12 * scale and imag > 0.1 * scale
+        else:              # cos cos or sin sin: real coefficient
+            assert imag <= 1e-12 * scale and real > 0.1 * scale
+        # x profile: the mode's coefficient is proportional to the band shape
+        prof = c32.real if py + pz != 1 else c32.imag
+        nzero = np.abs(sx) > 1e-9
+        ratio = prof[nzero] / sx[nzero]
+        assert np.abs(ratio - ratio[0]).max() <= 1e-12 * np.abs(ratio[0])
+    # parity about the band centre X = pi (m <-> nx/4 - m, 1-based m = i - 3)
+    m = band + 1 - 3
+    q = nx // 4
+    pairs = [(a, int(np.where(m == q - m[a])[0][0])) for a in range(len(m)) if 2 <= m[a] <= q - 2]
+    tol = [1e-12 * np.abs(c).max() for c in comps]
+    for a, b in pairs:
+        assert np.abs(comps[0][:, :, a] - comps[0][:, :, b]).max() <= tol[0]
+        assert np.abs(comps[1][:, :, a] + comps[1][:, :, b]).max() <= tol[1]
+        assert np.abs(comps[2][:, :, a] + comps[2][:, :, b]).max() <= tol[2]
