@@ -3,9 +3,11 @@
 
 Workload: the large single-GPU Re_tau = 180 channel, 256 x 192 x 256 cells,
 tanh-stretched wall-normal (y) mesh, synthetic log-law + Fourier initial
-state, constant-flow-rate forcing -- stepped in FP32 (value), FP32 compute
-with FP16-deviation storage, and FP64.  A "step" is one DUGKS time step of
-the whole grid.  Inputs are resident in HBM before timing starts.
+state, constant-flow-rate forcing -- stepped in FP32 (value), in FP32
+arithmetic with FP16 non-equilibrium storage (FP32 rho, u and FP16
+f - feq) and with FP16 deviation storage, and in FP64.  A "step" is one
+DUGKS time step of the whole grid.  Inputs are resident in HBM before timing
+starts.
 
 Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
 """
@@ -20,7 +22,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "dugks-gpu_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD public spec
-BYTES_PER_VALUE = {"fp64": 8, "fp32": 4, "fp16": 2}
+# stored bytes of one cell's state: 19 values, or (fp16n) rho and u in FP32
+# plus 19 FP16 values
+BYTES_PER_CELL = {"fp64": 152, "fp32": 76, "fp16": 38, "fp16n": 54}
 METRIC = "MLUPS & %HBM peak: FP64 vs FP32/FP16-store at stated error; adjoint/fwd cost"
 
 
@@ -75,7 +79,7 @@ def field_errors(ref, got, np):
 
 def run_mode(dugks_amd, configs, torch, precision, args, np=None):
     prec = {"fp64": dugks_amd.PREC_FP64, "fp32": dugks_amd.PREC_FP32,
-            "fp16": dugks_amd.PREC_FP16S}[precision]
+            "fp16": dugks_amd.PREC_FP16S, "fp16n": dugks_amd.PREC_FP16N}[precision]
     kernel = {"auto": dugks_amd.KERNEL_AUTO, "fused": dugks_amd.KERNEL_FUSED,
               "twopass": dugks_amd.KERNEL_TWOPASS}[args.kernel]
     text = configs.large_channel(precision=precision)
@@ -95,8 +99,7 @@ def run_mode(dugks_amd, configs, torch, precision, args, np=None):
     # kernel time of the step kernel, hipEvents on the launch stream
     kt = eng.time_steps(min(args.steps, 50), istep0=args.warmup + args.steps + 1)
     torch.cuda.synchronize()
-    S = BYTES_PER_VALUE[precision]
-    alg_bytes = 38 * S * ncell  # read f once + write f once, 19 values each
+    alg_bytes = 2 * BYTES_PER_CELL[precision] * ncell  # read the state once + write it once
     mlups = ncell * args.steps / wall / 1e6
     achieved = alg_bytes / (kt["mean_ms"] * 1e-3) / 1e9
     s = eng.global_sums()
@@ -303,8 +306,9 @@ def study_channel(dugks_amd, configs, torch, np, steps):
     synthetic log-law + Fourier state): mean-profile, rms and Reynolds-stress
     errors of FP32 and FP16-store against FP64 after `steps` steps."""
     stats, rates = {}, {}
-    for m in ("fp64", "fp32", "fp16"):
-        prec = {"fp64": dugks_amd.PREC_FP64, "fp32": dugks_amd.PREC_FP32, "fp16": dugks_amd.PREC_FP16S}[m]
+    for m in ("fp64", "fp32", "fp16n", "fp16"):
+        prec = {"fp64": dugks_amd.PREC_FP64, "fp32": dugks_amd.PREC_FP32, "fp16": dugks_amd.PREC_FP16S,
+                "fp16n": dugks_amd.PREC_FP16N}[m]
         eng = dugks_amd.Engine(configs.turbulent_channel(precision=m, nsteps=steps), precision=prec, device=0,
                                stream=torch.cuda.current_stream().cuda_stream)
         t0 = time.perf_counter()
@@ -320,7 +324,7 @@ def study_channel(dugks_amd, configs, torch, np, steps):
         eng.close()
     out = {"workload": "Re_tau=180 channel 128^3, tanh y (alpha 0.85), constant flow rate", "steps": steps,
            "against": "fp64", "mlups": {k: round(v, 1) for k, v in rates.items()}}
-    for m in ("fp32", "fp16"):
+    for m in ("fp32", "fp16n", "fp16"):
         out[m] = {k: float(f"{v:.3e}") for k, v in field_errors(stats["fp64"], stats[m], np).items()}
     return out
 
@@ -403,7 +407,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--modes", default="fp32,fp16,fp64")
+    ap.add_argument("--modes", default="fp32,fp16n,fp16,fp64")
     ap.add_argument("--kernel", default="auto", choices=["auto", "fused", "twopass"])
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="OpenMP threads of the CPU baseline (0: one per physical core of this "
@@ -490,7 +494,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "fp16": "f32-compute/f16-store"}[primary],
+        "dtype": {"fp64": "f64", "fp32": "f32", "fp16": "f32-compute/f16-store",
+                  "fp16n": "f32-compute/f32-moments+f16-neq-store"}[primary],
         "data": "synthetic",
         "config": {"workload": "Re_tau=180 channel 256x192x256, tanh-stretched y (alpha 0.85), "
                                "constant flow rate, log-law + Fourier initial state",

@@ -297,10 +297,7 @@ __global__ void __launch_bounds__(256) k_adj_cells(KParams P, const Store* __res
   });
   // S1^T
   Real ff[Q];
-  static_for<Q>([&](auto I) {
-    constexpr int ip = decltype(I)::value;
-    ff[ip] = Codec<Store>::template load<Real>(f, ip * P.N + c, P, ip);
-  });
+  load_cell<Real, Store>(f, c, P, ff);
   Real rho = 0, j[3] = {0, 0, 0};
   static_for<Q>([&](auto I) {
     constexpr int ip = decltype(I)::value;
@@ -766,6 +763,8 @@ hipError_t adjoint_step(Ctx& c, const void* f, const Lam* lam_next, Lam* lam_pre
       wall = a;
     }
   if (walls > 1) return hipErrorNotSupported;
+  // FP16 non-equilibrium storage has no adjoint
+  if (c.precision == 3) return hipErrorNotSupported;
   if constexpr (!std::is_same<Lam, double>::value) {
     if (c.precision == 0) return hipErrorNotSupported;
   }
@@ -800,7 +799,7 @@ hipError_t launch_adjoint_lam32(Ctx& c, const void* f, const float* lam_next, fl
 }
 
 bool adjoint_lam32_supported(const Ctx& c) {
-  if (c.precision == 0 || c.s.recon != 0) return false;
+  if (c.precision == 0 || c.precision == 3 || c.s.recon != 0) return false;
   return gather_z_supported(c) && adjoint_faces_fused_covers(c);
 }
 

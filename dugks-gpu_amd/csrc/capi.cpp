@@ -46,7 +46,16 @@ int fail(dugks_ctx* ctx, int code, const std::string& msg) {
     hipSetDevice(ctx->c.device);           \
   } while (0)
 
-size_t store_bytes(int precision) { return precision == 0 ? 8 : (precision == 1 ? 4 : 2); }
+// bytes of one stored state of N cells: 19 values per cell (FP64, FP32, FP16
+// deviations), or rho, u in FP32 and 19 FP16 values (FP16 non-equilibrium)
+size_t state_bytes_of(int precision, size_t N) {
+  switch (precision) {
+    case DUGKS_PREC_FP64: return N * Q * 8;
+    case DUGKS_PREC_FP32: return N * Q * 4;
+    case DUGKS_PREC_FP16S: return N * Q * 2;
+    default: return N * (4 * 4 + Q * 2);
+  }
+}
 size_t real_bytes(int precision) { return precision == 0 ? 8 : 4; }
 
 // Weight tables of one axis (layout documented on KParams).
@@ -95,10 +104,10 @@ int create(const Setup& s, const dugks_opts* opts, dugks_ctx** out) {
   dugks_default_opts(&o);
   if (opts) o = *opts;
   c.precision = o.precision >= 0 ? o.precision : s.precision;
-  if (c.precision < 0 || c.precision > 2) return fail(nullptr, DUGKS_EARG, "bad precision");
+  if (c.precision < 0 || c.precision > 3) return fail(nullptr, DUGKS_EARG, "bad precision");
   c.s.precision = c.precision;
   c.kernel = o.kernel >= 0 ? o.kernel : s.kernel;
-  c.store_bytes = store_bytes(c.precision);
+  c.state_bytes = state_bytes_of(c.precision, s.ncell());
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
     g_init_err = "no HIP device available";
@@ -130,8 +139,8 @@ int create(const Setup& s, const dugks_opts* opts, dugks_ctx** out) {
   }
   const size_t N = c.s.ncell();
   for (int b = 0; b < 2; b++)
-    if (c.alloc(&c.f[b], N * Q * c.store_bytes) != hipSuccess) {
-      g_init_err = "cannot allocate f (" + std::to_string(N * Q * c.store_bytes) + " bytes)";
+    if (c.alloc(&c.f[b], c.state_bytes) != hipSuccess) {
+      g_init_err = "cannot allocate f (" + std::to_string(c.state_bytes) + " bytes)";
       dugks_free(holder.release());
       return DUGKS_ENOMEM;
     }
@@ -779,10 +788,12 @@ int dugks_forward_record(dugks_ctx* ctx, int istep0, int T, int loss_kind, int i
   if (loss_kind == 0 && c.s.wall_axis() < 0) return fail(ctx, DUGKS_EUNSUPPORTED, "wall shear needs a wall axis");
   if (c.s.flow_rate != 0 || c.s.perturb)
     return fail(ctx, DUGKS_EUNSUPPORTED, "adjoint with flow-rate control or perturb forcing is not supported");
+  if (c.precision == DUGKS_PREC_FP16N)
+    return fail(ctx, DUGKS_EUNSUPPORTED, "the adjoint does not cover FP16 non-equilibrium storage");
   SweepPool& w = c.sweep;
   w.recorded = false;
   const size_t nf = c.s.ncell() * Q;
-  const size_t sb = c.store_bytes;
+  const size_t sb = c.state_bytes;
   if (interval <= 0) interval = (int)std::ceil(std::sqrt((double)T));
   interval = std::min(interval, T);
   const int ncp = (T + interval - 1) / interval;  // checkpoints at steps 0, K, 2K, ...
@@ -793,23 +804,23 @@ int dugks_forward_record(dugks_ctx* ctx, int istep0, int T, int loss_kind, int i
     return DUGKS_OK;
   };
   int rc;
-  if ((rc = chk(pool_states(c, w.cps, (size_t)ncp, nf * sb), "checkpoint alloc"))) return rc;
+  if ((rc = chk(pool_states(c, w.cps, (size_t)ncp, sb), "checkpoint alloc"))) return rc;
   if (!w.fT) {
-    if ((rc = chk(c.alloc(&w.fT, nf * sb), "final-state alloc"))) return rc;
-    w.bytes += (double)nf * sb;
+    if ((rc = chk(c.alloc(&w.fT, sb), "final-state alloc"))) return rc;
+    w.bytes += (double)sb;
   }
   // dval[0]: the loss summed over the forward sweep on the device, read once
   double* dval = c.red + reduction_scratch_doubles() + 9;
   if ((rc = chk(launch_fill(c, dval, 0.0, 1), "clear"))) return rc;
   for (int n = 0; n < T; n++) {
     if (n % interval == 0)
-      if ((rc = chk(hipMemcpyAsync(w.cps[n / interval], c.f[c.cur], nf * sb, hipMemcpyDeviceToDevice, c.stream),
+      if ((rc = chk(hipMemcpyAsync(w.cps[n / interval], c.f[c.cur], sb, hipMemcpyDeviceToDevice, c.stream),
                     "checkpoint copy")))
         return rc;
     if ((rc = step_once(ctx, istep0 + n))) return rc;
     if ((rc = chk(launch_loss(c, c.f[c.cur], loss_kind, dval, nullptr, 0, true), "loss"))) return rc;
   }
-  if ((rc = chk(hipMemcpyAsync(w.fT, c.f[c.cur], nf * sb, hipMemcpyDeviceToDevice, c.stream), "final copy")))
+  if ((rc = chk(hipMemcpyAsync(w.fT, c.f[c.cur], sb, hipMemcpyDeviceToDevice, c.stream), "final copy")))
     return rc;
   double Jsum = 0;
   if ((rc = chk(hipMemcpyAsync(&Jsum, dval, sizeof(double), hipMemcpyDeviceToHost, c.stream), "loss copy")))
@@ -833,7 +844,7 @@ int dugks_adjoint_replay(dugks_ctx* ctx, double* dev_grad_force, double* dev_gra
   if (!w.recorded) return fail(ctx, DUGKS_ESTATE, "no recorded forward sweep (dugks_forward_record)");
   const size_t N = c.s.ncell();
   const size_t nf = N * Q;
-  const size_t sb = c.store_bytes;
+  const size_t sb = c.state_bytes;
   const bool lam32 = sweep_lam32(c);
   const size_t lb = lam32 ? sizeof(float) : sizeof(double);
   const int T = w.T, K = w.interval;
@@ -845,7 +856,7 @@ int dugks_adjoint_replay(dugks_ctx* ctx, double* dev_grad_force, double* dev_gra
   };
   int rc;
   // segment states f^{n0+1} .. f^{n0+K-1}; f^{n0} is the checkpoint itself
-  if ((rc = chk(pool_states(c, w.seg, (size_t)(K - 1), nf * sb), "segment alloc"))) return rc;
+  if ((rc = chk(pool_states(c, w.seg, (size_t)(K - 1), sb), "segment alloc"))) return rc;
   if (w.lam_bytes < nf * lb) {
     for (int q = 0; q < 2; q++) {
       hipFree(w.lam[q]);

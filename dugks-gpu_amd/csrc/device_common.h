@@ -61,6 +61,27 @@ struct Codec<__half> {
   }
 };
 
+// FP16 non-equilibrium storage (the element tag of that mode's state arrays).
+// A state of N cells is one buffer: [4][N] float rho, u_x, u_y, u_z (u = j/rho,
+// the momentum velocity of f), then [Q][N] half s_i = n_i kNeqScale / w_i
+// with n_i = f_i - feq_i(rho, u).  The non-equilibrium part is small against
+// w_i, so FP16 rounds it far less than it would round f or f - w_i rho0; the
+// scale keeps typical |n_i| / w_i (1e-6 .. 1e-2) in FP16's normal range.
+struct NeqHalf {
+  unsigned short bits;
+};
+constexpr float kNeqScale = 1024.0f;
+// bytes of one state of N cells
+__host__ __device__ constexpr unsigned long long neq_state_bytes(unsigned long long N) { return N * (16 + 2 * 19); }
+// bytes of one state of N cells stored as Store
+template <class Store>
+__host__ __device__ constexpr unsigned long long state_bytes(unsigned long long N) {
+  if constexpr (std::is_same<Store, NeqHalf>::value)
+    return neq_state_bytes(N);
+  else
+    return N * 19 * sizeof(Store);
+}
+
 // Raw buffer access to a state array: one buffer resource in SGPRs, a 32-bit
 // per-lane byte offset and a uniform byte offset per population, so 19
 // population loads share one address register.  Requires the array to span
@@ -99,6 +120,25 @@ template <>
 __device__ __forceinline__ void StateBuf<__half>::put(__half v, unsigned vb, unsigned sb) const {
   __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(v), r, (int)vb, (int)sb, 0);
 }
+
+template <>
+struct StateBuf<NeqHalf> {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ StateBuf(const NeqHalf* p, unsigned bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<NeqHalf*>(p), 0, (int)bytes, 0x00020000)) {}
+  __device__ __forceinline__ __half raw(unsigned vb, unsigned sb) const {
+    return __ushort_as_half(__builtin_amdgcn_raw_buffer_load_b16(r, (int)vb, (int)sb, 0));
+  }
+  __device__ __forceinline__ void put(__half v, unsigned vb, unsigned sb) const {
+    __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(v), r, (int)vb, (int)sb, 0);
+  }
+  __device__ __forceinline__ float rawf(unsigned vb, unsigned sb) const {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vb, (int)sb, 0));
+  }
+  __device__ __forceinline__ void putf(float v, unsigned vb, unsigned sb) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)vb, (int)sb, 0);
+  }
+};
 
 // Raw buffer view of a plain array (< 2 GiB): per-lane 32-bit byte offset plus
 // a uniform byte offset; loads past the end return 0, stores past it are
@@ -293,17 +333,26 @@ struct EqTerm {
 // Centre-stage f-bar+ of one cell (calc_fb1 of the reference, src/dugks.cu:1131-1196):
 //   rho = sum f, u = j/rho + h F, feq_i(rho, u),
 //   fb_i = bw1 f_i + (bw2 + bw0 (c_i - u).F / RT) feq_i.
-// This form takes rho and j = sum c f of the cell (from cell_moments below).
+// This form takes rho and the momentum velocity u0 = j/rho of the cell (from
+// cell_moments and moment_velocity below, or from FP16 non-equilibrium
+// storage, which holds them).
 template <class Real>
-__device__ __forceinline__ void centre_fbar(const Real (&f)[Q], Real rho, const Real (&j)[3], const Real F[3],
+__device__ __forceinline__ void centre_fbar(const Real (&f)[Q], Real rho, const Real (&u0)[3], const Real F[3],
                                             Real hdt, Real bw0, Real bw1, Real bw2, Real (&fb)[Q]) {
-  const Real inv = rcp_real(rho);
-  const EqTerm<Real> eq(rho, j[0] * inv + hdt * F[0], j[1] * inv + hdt * F[1], j[2] * inv + hdt * F[2], F, bw2,
-                        bw0);
+  const EqTerm<Real> eq(rho, u0[0] + hdt * F[0], u0[1] + hdt * F[1], u0[2] + hdt * F[2], F, bw2, bw0);
   static_for<Q>([&](auto I) {
     constexpr int ip = decltype(I)::value;
     fb[ip] = eq.template relax<ip>(bw1, f[ip], cdot<ip>(F[0], F[1], F[2]));
   });
+}
+
+// u0 = j / rho
+template <class Real>
+__device__ __forceinline__ void moment_velocity(Real rho, const Real (&j)[3], Real (&u0)[3]) {
+  const Real inv = rcp_real(rho);
+  u0[0] = j[0] * inv;
+  u0[1] = j[1] * inv;
+  u0[2] = j[2] * inv;
 }
 
 // rho = sum_i f_i and j = sum_i c_i f_i of one cell, summed in population
@@ -323,9 +372,182 @@ __device__ __forceinline__ void cell_moments(const Real (&ff)[Q], Real& rho, Rea
 template <class Real>
 __device__ __forceinline__ void centre_fbar(const Real (&f)[Q], const Real F[3], Real hdt, Real bw0,
                                             Real bw1, Real bw2, Real (&fb)[Q]) {
-  Real rho, j[3];
+  Real rho, j[3], u0[3];
   cell_moments<Real>(f, rho, j);
-  centre_fbar<Real>(f, rho, j, F, hdt, bw0, bw1, bw2, fb);
+  moment_velocity<Real>(rho, j, u0);
+  centre_fbar<Real>(f, rho, u0, F, hdt, bw0, bw1, bw2, fb);
+}
+
+// ---- FP16 non-equilibrium storage (NeqHalf) ----
+
+// e_i(u) = 1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u.u, given base = 1 - 1.5 u.u
+template <int ip, class Real>
+__device__ __forceinline__ Real eq_shape(Real base, Real ux, Real uy, Real uz) {
+  if constexpr (ip == 0) {
+    return base;
+  } else {
+    const Real cu = cdot<ip>(ux, uy, uz);
+    return cu * (cu * Real(4.5) + Real(3)) + base;
+  }
+}
+
+// f_i = w_i rho e_i(u) + (w_i / kNeqScale) s_i
+template <class Real>
+__device__ __forceinline__ void neq_decode(Real rho, const Real (&u)[3], const __half (&s)[Q], Real (&ff)[Q]) {
+  const Real base = Real(1) - Real(1.5) * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  Real wr[3];
+  for (int k = 0; k < 3; k++) wr[k] = Real(wclass_weight(k)) * rho;
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    const Real e = eq_shape<ip>(base, u[0], u[1], u[2]);
+    ff[ip] = wr[wclass(ip)] * e + Real(wgt(ip) / (double)kNeqScale) * (Real)__half2float(s[ip]);
+  });
+}
+
+// rho, u = j/rho of f and s_i = (f_i - w_i rho e_i(u)) kNeqScale / w_i in FP16
+template <class Real>
+__device__ __forceinline__ void neq_encode(const Real (&ff)[Q], Real& rho, Real (&u)[3], __half (&s)[Q]) {
+  Real j[3];
+  cell_moments<Real>(ff, rho, j);
+  moment_velocity<Real>(rho, j, u);
+  const Real base = Real(1) - Real(1.5) * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  const Real rs = rho * Real(kNeqScale);
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    const Real e = eq_shape<ip>(base, u[0], u[1], u[2]);
+    s[ip] = __float2half((float)(ff[ip] * Real((double)kNeqScale / wgt(ip)) - rs * e));
+  });
+}
+
+// f of cell c (of N) in the arithmetic type, any storage.
+template <class Real, class Store>
+__device__ __forceinline__ void load_cell(const Store* f, long long c, const KParams& P, Real (&ff)[Q]) {
+  if constexpr (std::is_same<Store, NeqHalf>::value) {
+    const float* m = reinterpret_cast<const float*>(f);
+    const __half* s = reinterpret_cast<const __half*>(m + 4 * P.N);
+    __half sv[Q];
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      sv[ip] = s[ip * P.N + c];
+    });
+    const Real u[3] = {(Real)m[P.N + c], (Real)m[2 * P.N + c], (Real)m[3 * P.N + c]};
+    neq_decode<Real>((Real)m[c], u, sv, ff);
+  } else {
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      ff[ip] = Codec<Store>::template load<Real>(f, ip * P.N + c, P, ip);
+    });
+  }
+}
+
+// Stores f of cell c, any storage.
+template <class Real, class Store>
+__device__ __forceinline__ void store_cell(Store* f, long long c, const KParams& P, const Real (&ff)[Q]) {
+  if constexpr (std::is_same<Store, NeqHalf>::value) {
+    float* m = reinterpret_cast<float*>(f);
+    __half* s = reinterpret_cast<__half*>(m + 4 * P.N);
+    Real rho, u[3];
+    __half sv[Q];
+    neq_encode<Real>(ff, rho, u, sv);
+    m[c] = (float)rho;
+    m[P.N + c] = (float)u[0];
+    m[2 * P.N + c] = (float)u[1];
+    m[3 * P.N + c] = (float)u[2];
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      s[ip * P.N + c] = sv[ip];
+    });
+  } else {
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      Codec<Store>::template store<Real>(f, ip * P.N + c, ff[ip], P, ip);
+    });
+  }
+}
+
+// One cell's stored f as loaded through a StateBuf (the loads can be issued
+// ahead and decoded at use): the 19 stored values, plus rho and u for FP16
+// non-equilibrium storage.  Byte offsets: vb = cell index * element size,
+// population ip at ip * Nb (Nb = N * element size; NeqHalf: the FP16 part
+// follows the 16 N bytes of rho, u).
+template <class Real, class Store>
+struct RawCell {
+  Store v[Q];
+  __device__ __forceinline__ void load(const StateBuf<Store>& b, unsigned cell, unsigned N) {
+    const unsigned vb = cell * (unsigned)sizeof(Store), Nb = N * (unsigned)sizeof(Store);
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      v[ip] = b.raw(vb, ip * Nb);
+    });
+  }
+  // f, rho and the momentum velocity u0 = j/rho
+  __device__ __forceinline__ void decode(const KParams& P, Real (&ff)[Q], Real& rho, Real (&u0)[3]) const {
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      if constexpr (std::is_same<Store, __half>::value)
+        ff[ip] = (Real)((float)wgt(ip) * (P.rho0f + __half2float(v[ip])));
+      else
+        ff[ip] = (Real)v[ip];
+    });
+    Real j[3];
+    cell_moments<Real>(ff, rho, j);
+    moment_velocity<Real>(rho, j, u0);
+  }
+  // the rho and j only (FP32 storage path of the adjoint-face pass)
+  __device__ __forceinline__ void moments(const Real (&ff)[Q], Real& rho, Real (&j)[3]) const {
+    cell_moments<Real>(ff, rho, j);
+  }
+};
+template <class Real>
+struct RawCell<Real, NeqHalf> {
+  __half v[Q];
+  float m[4];
+  __device__ __forceinline__ void load(const StateBuf<NeqHalf>& b, unsigned cell, unsigned N) {
+    const unsigned vf = cell * 4u, vh = cell * 2u;
+    for (int q = 0; q < 4; q++) m[q] = b.rawf(vf, q * 4u * N);
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      v[ip] = b.raw(vh, 16u * N + ip * 2u * N);
+    });
+  }
+  __device__ __forceinline__ void decode(const KParams& P, Real (&ff)[Q], Real& rho, Real (&u0)[3]) const {
+    rho = (Real)m[0];
+    u0[0] = (Real)m[1];
+    u0[1] = (Real)m[2];
+    u0[2] = (Real)m[3];
+    neq_decode<Real>(rho, u0, v, ff);
+  }
+};
+
+// Stores the new f of a cell through a StateBuf; returns its rho and momentum
+// velocity u0 (needed for the encoding, and by the flow-rate partials) when
+// `want_u` or the storage needs them.
+template <class Real, class Store>
+__device__ __forceinline__ void store_raw_cell(const StateBuf<Store>& b, unsigned cell, unsigned N,
+                                               const KParams& P, const Real (&ff)[Q], bool want_u, Real& rho,
+                                               Real (&u0)[3]) {
+  if constexpr (std::is_same<Store, NeqHalf>::value) {
+    __half s[Q];
+    neq_encode<Real>(ff, rho, u0, s);
+    const unsigned vf = cell * 4u, vh = cell * 2u;
+    b.putf((float)rho, vf, 0);
+    for (int q = 0; q < 3; q++) b.putf((float)u0[q], vf, (q + 1) * 4u * N);
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      b.put(s[ip], vh, 16u * N + ip * 2u * N);
+    });
+  } else {
+    const unsigned vb = cell * (unsigned)sizeof(Store), Nb = N * (unsigned)sizeof(Store);
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      st_state<Real, Store>(b, vb, ip * Nb, ff[ip], P, ip);
+    });
+    if (want_u) {
+      Real j[3];
+      cell_moments<Real>(ff, rho, j);
+      moment_velocity<Real>(rho, j, u0);
+    }
+  }
 }
 
 // Adjoint of the equilibrium-with-force term (used by the adjoint of the

@@ -129,7 +129,7 @@ struct Ctx {
   bool own_stream = false;
   int precision = 0;
   int kernel = 0;
-  size_t store_bytes = 8;     // bytes per stored f value
+  size_t state_bytes = 0;     // bytes of one stored state (all cells)
   void* f[2] = {nullptr, nullptr};
   int cur = 0;                // f[cur] holds the current state
   void* fbx = nullptr;        // ghost-inclusive f-bar (two-pass kernels, adjoint)

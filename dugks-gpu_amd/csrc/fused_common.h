@@ -33,18 +33,17 @@ template <class Real, class Store>
 __device__ __forceinline__ void fbar_cell(const KParams& P, const StateBuf<Store>& f, unsigned Nb, int i, int j,
                                           int k, const Real Famb[3], Real (&fb)[Q]) {
   const unsigned c = ((unsigned)k * P.n[1] + j) * P.n[0] + i;
-  Real ff[Q];
-  static_for<Q>([&](auto I) {
-    constexpr int ip = decltype(I)::value;
-    ff[ip] = ld_state<Real, Store>(f, c * (unsigned)sizeof(Store), ip * Nb, P, ip);
-  });
+  RawCell<Real, Store> raw;
+  raw.load(f, c, (unsigned)P.N);
+  Real ff[Q], rho, u0[3];
+  raw.decode(P, ff, rho, u0);
   Real F[3] = {Famb[0], Famb[1], Famb[2]};
   if (P.fcell) {
     F[0] += (Real)P.fcell[c];
     F[1] += (Real)P.fcell[P.N + c];
     F[2] += (Real)P.fcell[2 * P.N + c];
   }
-  centre_fbar<Real>(ff, F, (Real)P.hdt, (Real)P.bw0, (Real)P.bw1, (Real)P.bw2, fb);
+  centre_fbar<Real>(ff, rho, u0, F, (Real)P.hdt, (Real)P.bw0, (Real)P.bw1, (Real)P.bw2, fb);
 }
 
 // f-bar+ of a column that may be a wall ghost in x and/or y: the ghost value

@@ -26,37 +26,41 @@ __global__ void __launch_bounds__(256) k_macro(KParams P, const Store* __restric
                                                double* ux, double* uy, double* uz) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= P.N) return;
-  double r = 0, u = 0, v = 0, w = 0;
-  static_for<Q>([&](auto I) {
-    constexpr int ip = decltype(I)::value;
-    const double x = Codec<Store>::template load<double>(f, ip * P.N + c, P, ip);
-    r += x;
-    u += CX[ip] * x;
-    v += CY[ip] * x;
-    w += CZ[ip] * x;
-  });
+  double x[Q], r, j[3];
+  load_cell<double, Store>(f, c, P, x);
+  cell_moments<double>(x, r, j);
   double F[3];
   ambient_d(P, F);
   rho[c] = r;
-  ux[c] = u / r + F[0] * P.hdt;
-  uy[c] = v / r + F[1] * P.hdt;
-  uz[c] = w / r + F[2] * P.hdt;
+  ux[c] = j[0] / r + F[0] * P.hdt;
+  uy[c] = j[1] / r + F[1] * P.hdt;
+  uz[c] = j[2] / r + F[2] * P.hdt;
 }
 
+// State -> FP64 SoA [19][N], one thread per cell.
 template <class Store>
 __global__ void __launch_bounds__(256) k_export(KParams P, const Store* __restrict__ f, double* dst) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= P.N * Q) return;
-  const int ip = (int)(t / P.N);
-  dst[t] = Codec<Store>::template load<double>(f, t, P, ip);
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= P.N) return;
+  double x[Q];
+  load_cell<double, Store>(f, c, P, x);
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    dst[ip * P.N + c] = x[ip];
+  });
 }
 
+// FP64 SoA [19][N] -> state, one thread per cell (a cell is encoded whole).
 template <class Store>
 __global__ void __launch_bounds__(256) k_import(KParams P, const double* __restrict__ src, Store* f) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= P.N * Q) return;
-  const int ip = (int)(t / P.N);
-  Codec<Store>::template store<double>(f, t, src[t], P, ip);
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= P.N) return;
+  double x[Q];
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    x[ip] = src[ip * P.N + c];
+  });
+  store_cell<double, Store>(f, c, P, x);
 }
 
 // Ops of block_reduce: NV sums.
@@ -87,15 +91,10 @@ __global__ void __launch_bounds__(kRedThreads) k_sums(KParams P, const Store* __
     int i, j, k;
     cell_coords(P, c, i, j, k);
     const double dV = P.wd[0][i + 1] * P.wd[1][j + 1] * P.wd[2][k + 1];
-    double r = 0, u = 0, v = 0, w = 0;
-    static_for<Q>([&](auto I) {
-      constexpr int ip = decltype(I)::value;
-      const double x = Codec<Store>::template load<double>(f, ip * P.N + c, P, ip);
-      r += x;
-      u += CX[ip] * x;
-      v += CY[ip] * x;
-      w += CZ[ip] * x;
-    });
+    double x[Q], r, jm[3];
+    load_cell<double, Store>(f, c, P, x);
+    cell_moments<double>(x, r, jm);
+    const double u = jm[0], v = jm[1], w = jm[2];
     acc[0] += r;
     acc[1] += u;
     acc[2] += v;
@@ -215,10 +214,7 @@ __global__ void __launch_bounds__(kRedThreads) k_loss(KParams P, const Store* __
       cell_coords(P, c, idx[0], idx[1], idx[2]);
     }
     double x[Q];
-    static_for<Q>([&](auto I) {
-      constexpr int ip = decltype(I)::value;
-      x[ip] = Codec<Store>::template load<double>(f, ip * P.N + c, P, ip);
-    });
+    load_cell<double, Store>(f, c, P, x);
     if (kind == 0) {
       const int m = idx[aw];
       const bool wall = m == 0 || m == P.n[aw] - 1;
@@ -243,14 +239,8 @@ __global__ void __launch_bounds__(kRedThreads) k_loss(KParams P, const Store* __
       acc[0] += val;
     } else {
       const double dV = P.wd[0][idx[0] + 1] * P.wd[1][idx[1] + 1] * P.wd[2][idx[2] + 1];
-      double r = 0, j[3] = {0, 0, 0};
-      static_for<Q>([&](auto I) {
-        constexpr int ip = decltype(I)::value;
-        r += x[ip];
-        j[0] += CX[ip] * x[ip];
-        j[1] += CY[ip] * x[ip];
-        j[2] += CZ[ip] * x[ip];
-      });
+      double r, j[3];
+      cell_moments<double>(x, r, j);
       double u[3], uu = 0;
       for (int a = 0; a < 3; a++) {
         u[a] = j[a] / r + P.hdt * F[a];
@@ -318,15 +308,9 @@ __global__ void __launch_bounds__(kRedThreads) k_stats(KParams P, const Store* _
     idx[a1] = (int)(s % P.n[a1]);
     idx[a2] = (int)(s / P.n[a1]);
     const long long c = ((long long)idx[2] * P.n[1] + idx[1]) * P.n[0] + idx[0];
-    double r = 0, j[3] = {0, 0, 0};
-    static_for<Q>([&](auto I) {
-      constexpr int ip = decltype(I)::value;
-      const double x = Codec<Store>::template load<double>(f, ip * P.N + c, P, ip);
-      r += x;
-      j[0] += CX[ip] * x;
-      j[1] += CY[ip] * x;
-      j[2] += CZ[ip] * x;
-    });
+    double x[Q], r, j[3];
+    load_cell<double, Store>(f, c, P, x);
+    cell_moments<double>(x, r, j);
     const double u = j[0] / r + F[0] * P.hdt, v = j[1] / r + F[1] * P.hdt, w = j[2] / r + F[2] * P.hdt;
     const double val[kStatSums] = {u, v, w, u * w, u * v, v * w, u * u, v * v, w * w, r, u, v, w};
     for (int q = 0; q < kStatSums; q++) acc[q] += val[q];
@@ -403,7 +387,7 @@ hipError_t macro_t(Ctx& c, const void* f, double* rho, double* ux, double* uy, d
 
 template <class Store>
 hipError_t export_t(Ctx& c, const void* f, double* dst) {
-  const unsigned nb = (unsigned)((c.kp.N * Q + 255) / 256);
+  const unsigned nb = (unsigned)((c.kp.N + 255) / 256);
   hipLaunchKernelGGL((k_export<Store>), dim3(nb), dim3(256), 0, c.stream, c.kp,
                      static_cast<const Store*>(f), dst);
   return hipGetLastError();
@@ -411,7 +395,7 @@ hipError_t export_t(Ctx& c, const void* f, double* dst) {
 
 template <class Store>
 hipError_t import_t(Ctx& c, const double* src, void* f) {
-  const unsigned nb = (unsigned)((c.kp.N * Q + 255) / 256);
+  const unsigned nb = (unsigned)((c.kp.N + 255) / 256);
   hipLaunchKernelGGL((k_import<Store>), dim3(nb), dim3(256), 0, c.stream, c.kp, src,
                      static_cast<Store*>(f));
   return hipGetLastError();
@@ -450,31 +434,36 @@ size_t stats_scratch_doubles(int np) { return (size_t)np * kStatBlocks * (kStatS
 hipError_t launch_stats(Ctx& c, const void* f, int pa, double* scratch, double* out) {
   if (c.precision == 0) return stats_t<double>(c, f, pa, scratch, out);
   if (c.precision == 1) return stats_t<float>(c, f, pa, scratch, out);
-  return stats_t<__half>(c, f, pa, scratch, out);
+  if (c.precision == 2) return stats_t<__half>(c, f, pa, scratch, out);
+  return stats_t<NeqHalf>(c, f, pa, scratch, out);
 }
 
 hipError_t launch_macro(Ctx& c, const void* f, double* rho, double* ux, double* uy, double* uz) {
   if (c.precision == 0) return macro_t<double>(c, f, rho, ux, uy, uz);
   if (c.precision == 1) return macro_t<float>(c, f, rho, ux, uy, uz);
-  return macro_t<__half>(c, f, rho, ux, uy, uz);
+  if (c.precision == 2) return macro_t<__half>(c, f, rho, ux, uy, uz);
+  return macro_t<NeqHalf>(c, f, rho, ux, uy, uz);
 }
 
 hipError_t launch_export(Ctx& c, const void* f, double* dst) {
   if (c.precision == 0) return export_t<double>(c, f, dst);
   if (c.precision == 1) return export_t<float>(c, f, dst);
-  return export_t<__half>(c, f, dst);
+  if (c.precision == 2) return export_t<__half>(c, f, dst);
+  return export_t<NeqHalf>(c, f, dst);
 }
 
 hipError_t launch_import(Ctx& c, const double* src, void* f) {
   if (c.precision == 0) return import_t<double>(c, src, f);
   if (c.precision == 1) return import_t<float>(c, src, f);
-  return import_t<__half>(c, src, f);
+  if (c.precision == 2) return import_t<__half>(c, src, f);
+  return import_t<NeqHalf>(c, src, f);
 }
 
 hipError_t launch_global_sums(Ctx& c, const void* f, double* dev_out8) {
   if (c.precision == 0) return sums_t<double>(c, f, dev_out8);
   if (c.precision == 1) return sums_t<float>(c, f, dev_out8);
-  return sums_t<__half>(c, f, dev_out8);
+  if (c.precision == 2) return sums_t<__half>(c, f, dev_out8);
+  return sums_t<NeqHalf>(c, f, dev_out8);
 }
 
 hipError_t launch_flow_control(Ctx& c, const void* f) {
@@ -498,14 +487,16 @@ hipError_t launch_loss(Ctx& c, const void* f, int kind, double* dev_value, doubl
                        bool value_acc) {
   if (c.precision == 0) return loss_t<double, double>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
   if (c.precision == 1) return loss_t<float, double>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
-  return loss_t<__half, double>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
+  if (c.precision == 2) return loss_t<__half, double>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
+  return loss_t<NeqHalf, double>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
 }
 
 hipError_t launch_loss_lam32(Ctx& c, const void* f, int kind, double* dev_value, float* dev_grad, double grad_acc,
                              bool value_acc) {
   if (c.precision == 0) return hipErrorNotSupported;
   if (c.precision == 1) return loss_t<float, float>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
-  return loss_t<__half, float>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
+  if (c.precision == 2) return loss_t<__half, float>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
+  return hipErrorNotSupported;
 }
 
 hipError_t launch_axpy(Ctx& c, double a, const double* x, double* y, long long n) {

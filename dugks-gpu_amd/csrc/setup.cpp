@@ -245,8 +245,10 @@ Setup make_setup(const Config& cfg, const std::string& base_dir) {
     s.precision = DUGKS_PREC_FP32;
   else if (prec == "fp16" || prec == "fp16s")
     s.precision = DUGKS_PREC_FP16S;
+  else if (prec == "fp16n")
+    s.precision = DUGKS_PREC_FP16N;
   else
-    throw ConfigError(DUGKS_ECONFIG, "precision must be fp64, fp32 or fp16");
+    throw ConfigError(DUGKS_ECONFIG, "precision must be fp64, fp32, fp16 or fp16n");
   std::string rc = cfg.str("recon", "tri");
   if (rc == "tri")
     s.recon = 0;

@@ -92,7 +92,8 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
   const int z0 = bz * TZ;
   const int zend = min(z0 + TZ, nz);
   const unsigned Nb = (unsigned)(P.N * (long long)sizeof(Store));  // bytes per population
-  const StateBuf<Store> bin(fin, Q * Nb), bout(fout, Q * Nb);
+  const unsigned span = (unsigned)state_bytes<Store>((unsigned long long)P.N);
+  const StateBuf<Store> bin(fin, span), bout(fout, span);
 
   // column of this thread in tile coordinates: r = 0..TY+1 (y = j0-1+r),
   // c = -1..TX (x = i0+c).  Lanes: the TY interior rows first (whole waves
@@ -244,14 +245,8 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
   // Columns inside the domain (tile and periodic halo) compute f-bar+ from f
   // of their own column; wall ghost columns extrapolate it (fbar_column).
   const bool own = active && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
-  // f of the own column at plane k
-  auto load_own = [&](int k, Real (&ff)[Q]) {
-    const unsigned vb = (((unsigned)k * ny + gj) * nx + gi) * (unsigned)sizeof(Store);
-    static_for<Q>([&](auto I) {
-      constexpr int ip = decltype(I)::value;
-      ff[ip] = ld_state<Real, Store>(bin, vb, ip * Nb, P, ip);
-    });
-  };
+  // stored f of the own column at plane k (decoded at use)
+  auto load_own = [&](int k, RawCell<Real, Store>& raw) { raw.load(bin, ((unsigned)k * ny + gj) * nx + gi, (unsigned)P.N); };
   // body force at plane k of the own column
   auto force_own = [&](int k, Real (&F)[3]) {
     F[0] = Famb[0];
@@ -281,11 +276,12 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
     if (active) {
       fbar_column<Real, Store>(P, bin, Nb, gi, gj, km, Famb, fbn);
       if (!ADJ && own) {
-        Real ff[Q], F[3], rho, jm[3];
-        load_own(z0, ff);
+        RawCell<Real, Store> raw;
+        Real ff[Q], F[3], rho, u0[3];
+        load_own(z0, raw);
+        raw.decode(P, ff, rho, u0);
         force_own(z0, F);
-        cell_moments<Real>(ff, rho, jm);
-        centre_fbar<Real>(ff, rho, jm, F, hdt, (Real)P.bw0, (Real)P.bw1, (Real)P.bw2, fbc);
+        centre_fbar<Real>(ff, rho, u0, F, hdt, (Real)P.bw0, (Real)P.bw1, (Real)P.bw2, fbc);
         if constexpr (!ADJ) form_g(fbc, ff, gc);
       } else {
         fbar_column<Real, Store>(P, bin, Nb, gi, gj, z0, Famb, fbc);
@@ -304,10 +300,10 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
   }
 
   // f of the own column one plane ahead of its f-bar: the loads are issued
-  // one iteration before their f-bar needs them (FP16 storage gains nothing
-  // from it and loads at use)
+  // one iteration before their f-bar needs them (FP16 deviation storage gains
+  // nothing from it and loads at use)
   constexpr bool kPrefetch = !std::is_same<Store, __half>::value;
-  Real fpre[Q];
+  RawCell<Real, Store> fpre;
   if (kPrefetch && own) load_own((z0 + 1) % nz, fpre);
   // compact adjoint-face records with FP32 storage: rho and j of every cell
   // for the gather, written by the interior lanes from the prefetched f.
@@ -329,14 +325,18 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
       if constexpr (!kPrefetch) load_own(kn, fpre);
       Real F[3];
       force_own(kn, F);
-      Real rho, jm[3];
-      cell_moments<Real>(fpre, rho, jm);
-      if constexpr (ADJ && COMPACT) {
-        if (interior) store_moments(kn, rho, jm);
+      Real ff[Q], rho, u0[3];
+      fpre.decode(P, ff, rho, u0);
+      if constexpr (ADJ && COMPACT && std::is_same<Store, float>::value) {
+        if (interior) {
+          Real jm[3];
+          cell_moments<Real>(ff, rho, jm);
+          store_moments(kn, rho, jm);
+        }
       }
-      centre_fbar<Real>(fpre, rho, jm, F, hdt, (Real)P.bw0, (Real)P.bw1, (Real)P.bw2, fbn);
+      centre_fbar<Real>(ff, rho, u0, F, hdt, (Real)P.bw0, (Real)P.bw1, (Real)P.bw2, fbn);
       if constexpr (!ADJ) {
-        if (interior) form_g(fbn, fpre, gn);
+        if (interior) form_g(fbn, ff, gn);
       }
       if (kPrefetch && p + 1 < zend) load_own((p + 2) % nz, fpre);
     } else if (active) {
@@ -505,6 +505,7 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
         const unsigned vb = cell * (unsigned)sizeof(Store);
         const Real updx = WX[5][c + 1], updy = WY[5][r];
         // f~(p+1) = 4/3 f-bar+ - 1/3 f~ - sum_a (dt/dx_a) c_a (F_a+ - F_a-)
+        constexpr bool kNeq = std::is_same<Store, NeqHalf>::value;
         Real fnew[Q];
         static_for<Q>([&](auto I) {
           constexpr int ip = decltype(I)::value;
@@ -516,17 +517,22 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
             v -= Real(CZ[ip]) * updz * (zp[kz] - zm[kz]);
             zm[kz] = zp[kz];
           }
-          st_state<Real, Store>(bout, vb, ip * Nb, v, P, ip);
+          // per-population stores go out as each value is ready; FP16
+          // non-equilibrium storage encodes the whole cell below
+          if constexpr (!kNeq) st_state<Real, Store>(bout, vb, ip * Nb, v, P, ip);
           fnew[ip] = v;
         });
-        if (partial) {
-          // bulk velocity of the new state for the flow-rate controller
-          // (per-cell moments in the arithmetic type, the sum over cells in FP64)
-          Real rr, jj[3];
+        Real rr, u0n[3];
+        if constexpr (kNeq) {
+          store_raw_cell<Real, Store>(bout, cell, (unsigned)P.N, P, fnew, true, rr, u0n);
+        } else if (partial) {
+          Real jj[3];
           cell_moments<Real>(fnew, rr, jj);
-          const Real js = as == 0 ? jj[0] : (as == 1 ? jj[1] : jj[2]);
-          flow += (double)(js * rcp_real(rr));  // u along the flow axis; see k_flow_control
+          moment_velocity<Real>(rr, jj, u0n);
         }
+        // bulk velocity of the new state for the flow-rate controller
+        // (per-cell moments in the arithmetic type, the sum over cells in FP64)
+        if (partial) flow += (double)(as == 0 ? u0n[0] : (as == 1 ? u0n[1] : u0n[2]));
       }
     }
     if (active) {
@@ -630,7 +636,7 @@ bool fused_supported(const Ctx& c, int* tx, int* ty) {
   const Setup& s = c.s;
   if (s.recon != 0 || !s.periodic[2] || !s.uniform[2]) return false;
   // buffer offsets are 32-bit: one state array must span < 4 GiB
-  if ((double)s.ncell() * Q * c.store_bytes >= 4294967296.0) return false;
+  if ((double)c.state_bytes >= 4294967296.0) return false;
   // Tiles of 4 waves: 16 x 12 cells when the grid divides (1.31 f-bar
   // evaluations per cell, the x-halo lanes packed behind the rows), else
   // 32 x 4 (2.0 per cell).  Four waves per block keep FP64 at one wave per
@@ -661,14 +667,17 @@ hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled)
       return tall ? run_fused<double, double, 16, 12>(c, fin, fout) : run_fused<double, double, 32, 4>(c, fin, fout);
     case 1:
       return tall ? run_fused<float, float, 16, 12>(c, fin, fout) : run_fused<float, float, 32, 4>(c, fin, fout);
-    default:
+    case 2:
       return tall ? run_fused<float, __half, 16, 12>(c, fin, fout) : run_fused<float, __half, 32, 4>(c, fin, fout);
+    default:
+      return tall ? run_fused<float, NeqHalf, 16, 12>(c, fin, fout) : run_fused<float, NeqHalf, 32, 4>(c, fin, fout);
   }
 }
 
 bool adjoint_faces_fused_covers(const Ctx& c) {
   int tx = 32, ty = 4;
-  if (c.kernel == 0 || !fused_supported(c, &tx, &ty)) return false;
+  // the adjoint does not cover FP16 non-equilibrium storage
+  if (c.kernel == 0 || c.precision == 3 || !fused_supported(c, &tx, &ty)) return false;
   // lam' and each face array are addressed with 32-bit buffer offsets
   const double rb = c.precision == 0 ? 8.0 : 4.0;
   return (double)Q * c.kp.N * 8.0 < 2147483648.0 && (double)Q * c.kp.eN * rb < 2147483648.0;

@@ -40,10 +40,7 @@ __global__ void __launch_bounds__(256) k_fbar(KParams P, const Store* __restrict
   const int k = (int)(t / P.n[1]);
   const long long e = ((long long)(k + 1) * P.e[1] + (j + 1)) * P.e[0] + (i + 1);
   Real ff[Q], fb[Q];
-  static_for<Q>([&](auto I) {
-    constexpr int ip = decltype(I)::value;
-    ff[ip] = Codec<Store>::template load<Real>(f, ip * P.N + c, P, ip);
-  });
+  load_cell<Real, Store>(f, c, P, ff);
   Real F[3];
   ambient(P, F);
   if (P.fcell) {
@@ -195,10 +192,10 @@ __global__ void __launch_bounds__(256) k_flux(KParams P, const Store* __restrict
   const Real hdt = (Real)P.hdt, fw0 = (Real)P.fw0, fw1 = (Real)P.fw1, fw2 = (Real)P.fw2;
   Real acc[Q];
   // f~+ = 4/3 f-bar+ - 1/3 f~ (src/dugks.cu:954)
+  load_cell<Real, Store>(fin, c, P, acc);
   static_for<Q>([&](auto I) {
     constexpr int ip = decltype(I)::value;
-    const Real fo = Codec<Store>::template load<Real>(fin, ip * P.N + c, P, ip);
-    acc[ip] = fbx[ip * P.eN + e] * Real(4.0 / 3.0) - fo * Real(1.0 / 3.0);
+    acc[ip] = fbx[ip * P.eN + e] * Real(4.0 / 3.0) - acc[ip] * Real(1.0 / 3.0);
   });
   auto face = [&](auto Ac, auto Sc) {
     constexpr int A = decltype(Ac)::value;
@@ -232,10 +229,7 @@ __global__ void __launch_bounds__(256) k_flux(KParams P, const Store* __restrict
   face(std::integral_constant<int, 1>{}, std::integral_constant<int, 1>{});
   face(std::integral_constant<int, 2>{}, std::integral_constant<int, -1>{});
   face(std::integral_constant<int, 2>{}, std::integral_constant<int, 1>{});
-  static_for<Q>([&](auto I) {
-    constexpr int ip = decltype(I)::value;
-    Codec<Store>::template store<Real>(fout, ip * P.N + c, acc[ip], P, ip);
-  });
+  store_cell<Real, Store>(fout, c, P, acc);
 }
 
 template <class Real, class Store>
@@ -286,8 +280,10 @@ hipError_t launch_fbar_ghosts(Ctx& c, const void* fin, void* fbx) {
       return run_fbar_ghosts<double, double>(c, fin, fbx);
     case 1:
       return run_fbar_ghosts<float, float>(c, fin, fbx);
-    default:
+    case 2:
       return run_fbar_ghosts<float, __half>(c, fin, fbx);
+    default:
+      return run_fbar_ghosts<float, NeqHalf>(c, fin, fbx);
   }
 }
 
@@ -297,8 +293,10 @@ hipError_t launch_step_twopass(Ctx& c, const void* fin, void* fout) {
       return run_twopass<double, double>(c, fin, fout);
     case 1:
       return run_twopass<float, float>(c, fin, fout);
-    default:
+    case 2:
       return run_twopass<float, __half>(c, fin, fout);
+    default:
+      return run_twopass<float, NeqHalf>(c, fin, fout);
   }
 }
 

@@ -14,7 +14,7 @@ ERRORS = {
     -1: "DUGKS_EARG", -2: "DUGKS_ECONFIG", -3: "DUGKS_EIO", -4: "DUGKS_EHIP",
     -5: "DUGKS_ESTATE", -6: "DUGKS_EUNSUPPORTED", -7: "DUGKS_ENOMEM", -8: "DUGKS_ENUMERIC",
 }
-PREC_CONFIG, PREC_FP64, PREC_FP32, PREC_FP16S = -1, 0, 1, 2
+PREC_CONFIG, PREC_FP64, PREC_FP32, PREC_FP16S, PREC_FP16N = -1, 0, 1, 2, 3
 KERNEL_AUTO, KERNEL_TWOPASS, KERNEL_FUSED = -1, 0, 1
 LOSS_WALL_SHEAR, LOSS_KINETIC_ENERGY = 0, 1
 

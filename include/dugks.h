@@ -40,6 +40,8 @@ extern "C" {
 #define DUGKS_PREC_FP64 0      /* FP64 arithmetic, FP64 storage of f */
 #define DUGKS_PREC_FP32 1      /* FP32 arithmetic, FP32 storage of f */
 #define DUGKS_PREC_FP16S 2     /* FP32 arithmetic, FP16 storage of (f - w_i rho0)/w_i */
+#define DUGKS_PREC_FP16N 3     /* FP32 arithmetic, FP32 rho and u = j/rho plus FP16 storage of
+                                  (f - feq(rho, u)) * 1024 / w_i; no adjoint */
 
 /* Step kernels. */
 #define DUGKS_KERNEL_AUTO (-1)

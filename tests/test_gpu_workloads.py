@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 import dugks_amd
-from dugks_amd import PREC_FP16S, PREC_FP32, PREC_FP64, configs
+from dugks_amd import PREC_FP16N, PREC_FP16S, PREC_FP32, PREC_FP64, configs
 
 pytestmark = pytest.mark.gpu
 
@@ -117,10 +117,10 @@ _FP16S_MISSES = pytest.mark.xfail(strict=True, reason="FP16 deviation storage ro
                                                       "known to exceed the 1e-3 field-error bar here")
 
 
-@pytest.mark.parametrize("mode", ["fp32", pytest.param("fp16", marks=_FP16S_MISSES)])
+@pytest.mark.parametrize("mode", ["fp32", "fp16n", pytest.param("fp16", marks=_FP16S_MISSES)])
 def test_channel_128_storage_modes_against_fp64(channel_128, mode):
     steps, ref, ub0_ref, ub_ref = channel_128
-    prec = {"fp32": PREC_FP32, "fp16": PREC_FP16S}[mode]
+    prec = {"fp32": PREC_FP32, "fp16": PREC_FP16S, "fp16n": PREC_FP16N}[mode]
     eng = _engine(configs.turbulent_channel(precision=mode, nsteps=steps), prec)
     ub0 = _bulk(eng)
     eng.step(steps, istep0=1)
@@ -138,7 +138,7 @@ def test_channel_128_storage_modes_against_fp64(channel_128, mode):
 def test_large_channel_256x192x256():
     steps = 200
     out = {}
-    for name, prec in (("fp64", PREC_FP64), ("fp32", PREC_FP32), ("fp16", PREC_FP16S)):
+    for name, prec in (("fp64", PREC_FP64), ("fp32", PREC_FP32), ("fp16n", PREC_FP16N), ("fp16", PREC_FP16S)):
         eng = _engine(configs.large_channel(precision=name, nsteps=steps), prec)
         s0 = eng.global_sums()
         ub0 = _bulk(eng)
@@ -146,12 +146,13 @@ def test_large_channel_256x192x256():
         s1 = eng.global_sums()
         assert all(math.isfinite(v) for v in s1), name
         # mass: round-off of the arithmetic type (FP16 storage rounds each state)
-        tol = {"fp64": 1e-12, "fp32": 1e-5, "fp16": 1e-3}[name]
+        tol = {"fp64": 1e-12, "fp32": 1e-5, "fp16n": 1e-5, "fp16": 1e-3}[name]
         assert abs(s1[0] - s0[0]) / s0[0] <= tol, name
         assert abs(_bulk(eng) - ub0) <= 1e-5 * ub0, name
         if name != "fp16":
             out[name] = _velocity_stats(eng)
         eng.close()
-    err = _field_errors(out["fp64"], out["fp32"])
-    bad = {k: v for k, v in err.items() if not v <= FIELD_TOL}
-    assert not bad, err
+    for name in ("fp32", "fp16n"):
+        err = _field_errors(out["fp64"], out[name])
+        bad = {k: v for k, v in err.items() if not v <= FIELD_TOL}
+        assert not bad, (name, err)
