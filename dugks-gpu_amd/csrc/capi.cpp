@@ -516,6 +516,66 @@ int dugks_get_statistics(dugks_ctx* ctx, double* profiles, double diag[12], int*
   return DUGKS_OK;
 }
 
+int dugks_spectral_statistics(dugks_ctx* ctx, double* spectrum, int nbins, double out[13]) {
+  CHECK_CTX();
+  if (!out || nbins < 0 || (nbins > 0 && !spectrum)) return fail(ctx, DUGKS_EARG, "bad spectral statistics arguments");
+  Ctx& c = ctx->c;
+  const Setup& s = c.s;
+  for (int a = 0; a < 3; a++)
+    if (!s.periodic[a] || !s.uniform[a])
+      return fail(ctx, DUGKS_EUNSUPPORTED, "spectral statistics need periodic uniform axes");
+  if (s.width[1] != s.width[0] || s.width[2] != s.width[0])
+    return fail(ctx, DUGKS_EUNSUPPORTED, "spectral statistics need a cubic domain (equal widths)");
+  const int nb = spectral_bins(c);
+  std::vector<double> shells(2 * (size_t)nb);
+  double sums[4];
+  hipError_t e = spectral_statistics(c, shells.data(), sums);
+  if (e == hipErrorInvalidValue) return fail(ctx, DUGKS_EUNSUPPORTED, "grid too large for the shell sums");
+  if (e != hipSuccess) return fail(ctx, DUGKS_EHIP, std::string("spectral statistics: ") + hipGetErrorString(e));
+  // stat_te, src/fortran/dhit/src/stat.f90:170-197
+  double etotal = 0, eps = 0;
+  for (int b = 0; b < nb; b++) {
+    etotal += shells[b];
+    eps += shells[nb + b];
+  }
+  const double nu = s.visc, L = s.width[0], dx = L / s.n[0];
+  const double N3 = 3.0 * (double)s.ncell();
+  const double uprm = std::sqrt(2.0 / 3.0 * etotal);
+  const double tmse = std::sqrt(15 * nu * uprm * uprm / eps);
+  const double bottom = sums[1] / N3;
+  out[0] = sums[0];                                      // te
+  out[1] = etotal;                                       // sum of E(k)
+  out[2] = eps;                                          // 2 nu sum k^2 E(k)
+  out[3] = std::pow(nu * nu * nu / eps, 0.25);           // Kolmogorov length
+  out[4] = std::pow(nu * eps, 0.25);                     // Kolmogorov velocity
+  out[5] = std::sqrt(nu / eps);                          // Kolmogorov time
+  out[6] = uprm;                                         // u' = sqrt(2 etotal / 3)
+  out[7] = tmse;                                         // Taylor microscale
+  out[8] = uprm * tmse / nu;                             // Taylor-scale Reynolds number
+  out[9] = uprm * uprm * uprm / eps;                     // large-eddy length
+  out[10] = (L / 2 - 1.5 * dx) * 2 * M_PI / L * out[3];  // k_max eta
+  out[11] = (sums[2] / N3) / std::pow(bottom, 1.5);      // derivative skewness
+  out[12] = (sums[3] / N3) / (bottom * bottom);          // derivative flatness
+  for (int b = 0; b < nbins; b++) spectrum[b] = b < nb ? shells[b] : 0.0;
+  return DUGKS_OK;
+}
+
+int dugks_write_spectral_outputs(dugks_ctx* ctx, int step, const char* dir) {
+  CHECK_CTX();
+  const Setup& s = ctx->c.s;
+  const int nspec = s.n[0] + 1;  // shells 0..nx, as the reference writes them
+  std::vector<double> spec(nspec);
+  double st[13];
+  int rc = dugks_spectral_statistics(ctx, spec.data(), nspec, st);
+  if (rc) return rc;
+  try {
+    write_spectral_outputs(step, s.dt, st, spec.data(), nspec, dir ? dir : "");
+  } catch (const ConfigError& e) {
+    return fail(ctx, e.code, e.what());
+  }
+  return DUGKS_OK;
+}
+
 int dugks_write_outputs(dugks_ctx* ctx, int step, const char* dir) {
   CHECK_CTX();
   const Setup& s = ctx->c.s;

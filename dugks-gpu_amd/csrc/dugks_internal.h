@@ -164,6 +164,13 @@ struct Ctx {
   }
 };
 
+// Spectral (DHIT) statistics of the current state, spectral.hip: the number
+// of wavenumber shells of the grid, and shells [2][nb] (E, 2 nu k^2 E) plus
+// sums[4] (physical 0.5 <|u|^2>; sums of d^2, d^3, d^4 of the longitudinal
+// derivatives over the three components) into host buffers.
+int spectral_bins(const Ctx& c);
+hipError_t spectral_statistics(Ctx& c, double* shells, double sums[4]);
+
 // Builds the kernel parameter block and the device tables for a setup.
 hipError_t build_params(Ctx& c);
 

@@ -153,6 +153,35 @@ void finish_statistics(const Setup& s, const double* sums, double* profiles, dou
   }
 }
 
+void write_spectral_outputs(int step, double dt, const double st[13], const double* spectrum, int nspec,
+                            const std::string& dir) {
+  // Fortran formats I7, F16.8, 1pe15.6 and I4 with 1X separators
+  auto open_append = [&](const char* name) {
+    FILE* f = std::fopen(join_path(dir, name).c_str(), "a");
+    if (!f) throw ConfigError(DUGKS_EIO, std::string("cannot append ") + name);
+    return f;
+  };
+  const double t = step * dt;
+  FILE* f = open_append("stat_te.dat");
+  std::fprintf(f, "%7d %16.8f %15.6E\n", step, t, st[0]);
+  std::fclose(f);
+  // etotal, ediss_total0, ediss_total (written as 0 by the reference), eta,
+  // vk, tk, uprm, tmse, Retms, xl, kmxeta
+  f = open_append("stat_te_spectral.dat");
+  std::fprintf(f, "%7d %16.8f %15.6E %15.6E %15.6E %15.6E %15.6E %15.6E %15.6E %15.6E %15.6E %15.6E %15.6E \n", step,
+               t, st[1], st[2], 0.0, st[3], st[4], st[5], st[6], st[7], st[8], st[9], st[10]);
+  std::fclose(f);
+  char name[32];
+  std::snprintf(name, sizeof name, "spe%9.9d.dat", step);
+  f = std::fopen(join_path(dir, name).c_str(), "w");
+  if (!f) throw ConfigError(DUGKS_EIO, std::string("cannot write ") + name);
+  for (int i = 0; i < nspec; i++) std::fprintf(f, "%4d %15.6E\n", i, spectrum[i]);
+  std::fclose(f);
+  f = open_append("skew-flat.dat");
+  std::fprintf(f, "%7d %15.6E %15.6E %15.6E\n", step, t, st[11], st[12]);
+  std::fclose(f);
+}
+
 // A double as Python's repr() writes it (the VELMEAN.dat rows are Python
 // f-strings): the shortest digit string that reads back to the same double,
 // in positional notation with at least one fractional digit ("1.0",

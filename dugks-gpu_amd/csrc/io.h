@@ -34,6 +34,13 @@ int stats_axis(const Setup& s);
 void finish_statistics(const Setup& s, const double* sums, double* profiles, double diag[12]);
 
 // VELMEAN.dat from profiles-2.dat (examples/channel/ProcessProfiles.py:17-79).
+// DHIT statistics files of stat_te (src/fortran/dhit/src/stat.f90:58-198),
+// appended / written in dir: stat_te.dat, stat_te_spectral.dat,
+// spe%09d.dat (shells 0..nspec-1) and skew-flat.dat.  stats is the
+// dugks_spectral_statistics vector.
+void write_spectral_outputs(int step, double dt, const double stats[13], const double* spectrum, int nspec,
+                            const std::string& dir);
+
 void velmean(const std::string& in_path, int n, int ignore_before, int mean_col, int stress_col,
              const std::string& out_path);
 

@@ -50,6 +50,8 @@ PROTOTYPES = {
     "dugks_time_steps": (_i, [_ctxp, _i, _i, _dp]),
     "dugks_get_dims": (_i, [_ctxp, _ip]),
     "dugks_get_device": (_i, [_ctxp, _ip]),
+    "dugks_spectral_statistics": (_i, [_ctxp, _dp, _i, _dp]),
+    "dugks_write_spectral_outputs": (_i, [_ctxp, _i, _cp]),
     "dugks_get_params": (_i, [_ctxp, _dp]),
     "dugks_get_schedule": (_i, [_ctxp, _ip]),
     "dugks_write_outputs": (_i, [_ctxp, _i, _cp]),

@@ -218,6 +218,22 @@ class Engine:
         self._check(self._lib.dugks_get_statistics(self._h, _ptr(prof), _ptr(diag), ctypes.byref(rows)))
         return prof[:rows.value].copy(), diag
 
+    SPECTRAL_KEYS = ("te", "etotal", "eps", "eta", "vk", "tk", "uprime", "taylor", "re_lambda", "L",
+                     "kmax_eta", "skewness", "flatness")
+
+    def spectral_statistics(self, nbins=None):
+        """DHIT statistics of the current velocity field (stat_te of the
+        reference): (dict of the SPECTRAL_KEYS values, E(k) of shells
+        0..nbins-1); nbins defaults to nx + 1."""
+        nb = self.shape[0] + 1 if nbins is None else int(nbins)
+        spec = np.zeros(max(nb, 1))
+        out = np.empty(13)
+        self._check(self._lib.dugks_spectral_statistics(self._h, _ptr(spec), nb, _ptr(out)))
+        return dict(zip(self.SPECTRAL_KEYS, out.tolist())), spec[:nb]
+
+    def write_spectral_outputs(self, step, dirname="."):
+        self._check(self._lib.dugks_write_spectral_outputs(self._h, int(step), _b(dirname)))
+
     # -- adjoint --
     def loss(self, kind=_lib.LOSS_WALL_SHEAR, grad=None):
         v = ctypes.c_double()

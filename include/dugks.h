@@ -126,6 +126,28 @@ int dugks_write_outputs(dugks_ctx* ctx, int step, const char* dir);
  * diag.dat columns).  Host buffers; synchronises the stream. */
 int dugks_get_statistics(dugks_ctx* ctx, double* profiles, double diag[12], int* rows);
 
+/* Spectral statistics of decaying homogeneous isotropic turbulence, stat_te
+ * of the reference's DHIT code (src/fortran/dhit/src/stat.f90:5-205), of the
+ * current velocity field (the output velocity of dugks_get_macro), on the
+ * device with hipFFT and FP64 sums.  Needs periodic uniform axes and a cubic
+ * domain.  The fluctuation u' = u - <u> is transformed (FFT normalised by
+ * N); shell k holds the modes with nint(|kappa|) = k, kappa the integer
+ * wavenumber vector (wavenumber 2 pi kappa / L).  out[13]:
+ *   te = 0.5 <|u|^2> (physical space, mean included), etotal = sum_k E(k),
+ *   eps = 2 nu sum |k|^2 E, Kolmogorov length, velocity and time,
+ *   u' = sqrt(2 etotal / 3), Taylor microscale sqrt(15 nu u'^2 / eps),
+ *   Re_lambda, large-eddy length u'^3 / eps, k_max eta with
+ *   k_max = (L/2 - 1.5 dx) 2 pi / L, and the skewness and flatness of the
+ *   longitudinal derivatives du/dx, dv/dy, dw/dz (spectral derivatives).
+ * spectrum: E(k) of shells 0..nbins-1 (may be NULL when nbins = 0).
+ * Synchronises.  Differences from the reference: exact pi (it uses
+ * 3.14159), FP64 sums (it sums in FP32). */
+int dugks_spectral_statistics(dugks_ctx* ctx, double* spectrum, int nbins, double out[13]);
+/* The stat_te files in dir (NULL = current): appends to stat_te.dat,
+ * stat_te_spectral.dat and skew-flat.dat and writes spe%09d.dat (shells
+ * 0..nx), in the reference's formats (stat.f90:58-198). */
+int dugks_write_spectral_outputs(dugks_ctx* ctx, int step, const char* dir);
+
 /* Cell widths of one axis (n values); centres are cumulative midpoints from
  * the low wall/edge at coordinate 0. */
 int dugks_get_spacing(const dugks_ctx* ctx, int axis, double* widths);

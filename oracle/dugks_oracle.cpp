@@ -28,6 +28,15 @@
 // The adjoint (oracle_adjoint_step) has no reference counterpart; it is the
 // reverse-mode derivative of exactly this forward step, stage by stage.
 //
+// Licence.  This file restates, line for line, algorithms of
+// Kairzhan/DUGKS-GPU, which is distributed under the GNU General Public
+// License version 3.  As a derivative of that code this file is licensed under
+// the GNU General Public License, version 3 or (at your option) any later
+// version: you may redistribute and/or modify it under those terms.  It is
+// distributed WITHOUT ANY WARRANTY; without even the implied warranty of
+// MERCHANTABILITY or FITNESS FOR A PARTICULAR PURPOSE.  See
+// <https://www.gnu.org/licenses/> for the licence text.
+//
 // Compiled with -ffp-contract=off so every product and sum rounds as written.
 
 #include "dugks_oracle.h"
