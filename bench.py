@@ -28,19 +28,39 @@ BYTES_PER_CELL = {"fp64": 152, "fp32": 76, "fp16": 38, "fp16n": 54}
 METRIC = "MLUPS & %HBM peak: FP64 vs FP32/FP16-store at stated error; adjoint/fwd cost"
 
 
+# sources of the step kernel: a traffic figure measured on other sources is stale
+STEP_KERNEL_SOURCES = ("step_fused.hip", "fused_common.h", "device_common.h", "dugks_internal.h", "lattice.h")
+
+
+def step_kernel_hash():
+    """First 16 hex digits of the SHA-256 of the step kernel's sources."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in STEP_KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "dugks-gpu_amd", "csrc", name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(precision):
     """HBM bytes per launch of the step kernel on this workload, from the
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary committed under profiles/
-    (each counter scaled by its reading for a copy kernel of known size).
-    None when no summary for this precision exists."""
+    (each counter scaled by its reading for a copy kernel of known size),
+    with the source hash it was measured on: (bytes, hash, current).  bytes
+    is None when no summary for this precision exists or when it was measured
+    on other kernel sources than the current ones (current False)."""
     path = os.path.join(ROOT, "profiles", "k_fused_traffic_256x192x256.txt")
     if not os.path.exists(path):
-        return None
+        return None, None, False
+    value, stamp = None, None
     for line in open(path):
         parts = line.split()
+        if len(parts) >= 2 and parts[0] == "source_hash":
+            stamp = parts[1]
         if len(parts) >= 3 and parts[0] == "traffic" and parts[1] == precision:
-            return int(float(parts[2]))
-    return None
+            value = int(float(parts[2]))
+    current = stamp is not None and stamp == step_kernel_hash()
+    return (value if current else None), stamp, current
 
 
 def log(msg):
@@ -476,6 +496,7 @@ def main():
         adjoint = adjoint_leg(dugks_amd, configs, torch, args.adj_steps)
     primary = "fp32" if "fp32" in results else modes[0]
     r = results[primary]
+    traffic = measured_traffic(primary)
     cpu = None
     if not args.no_cpu:
         _, phys, share = host_cpu()
@@ -504,7 +525,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(primary)},
+                     "traffic": traffic[0],
+                     "traffic_source_hash": traffic[1], "kernel_source_hash": step_kernel_hash()},
         "cpu_baseline": cpu,
         "precision_errors": errors,
         "adjoint": adjoint,

@@ -512,7 +512,7 @@ int dugks_get_statistics(dugks_ctx* ctx, double* profiles, double diag[12], int*
   std::vector<double> sums((size_t)np * 9 + 12);
   HIPCHK(hipMemcpyAsync(sums.data(), out, sums.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   HIPCHK(hipStreamSynchronize(c.stream));
-  finish_statistics(c.s, sums.data(), profiles, diag);
+  finish_statistics(c.s, sums.data(), sums.data() + (size_t)np * 9, profiles, diag);
   return DUGKS_OK;
 }
 

@@ -107,6 +107,10 @@ template <>
 __device__ __forceinline__ __half StateBuf<__half>::raw(unsigned vb, unsigned sb) const {
   return __ushort_as_half(__builtin_amdgcn_raw_buffer_load_b16(r, (int)vb, (int)sb, 0));
 }
+// The 16 bits of an FP16 state value zero-extended to 32 (see RawCell).
+__device__ __forceinline__ unsigned raw_b16_32(const __amdgpu_buffer_rsrc_t& r, unsigned vb, unsigned sb) {
+  return (unsigned)__builtin_amdgcn_raw_buffer_load_b16(r, (int)vb, (int)sb, 0);
+}
 template <>
 __device__ __forceinline__ void StateBuf<double>::put(double v, unsigned vb, unsigned sb) const {
   typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
@@ -128,6 +132,10 @@ struct StateBuf<NeqHalf> {
       : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<NeqHalf*>(p), 0, (int)bytes, 0x00020000)) {}
   __device__ __forceinline__ __half raw(unsigned vb, unsigned sb) const {
     return __ushort_as_half(__builtin_amdgcn_raw_buffer_load_b16(r, (int)vb, (int)sb, 0));
+  }
+  // the 16 bits zero-extended to 32 (see RawCell)
+  __device__ __forceinline__ unsigned raw32(unsigned vb, unsigned sb) const {
+    return (unsigned)__builtin_amdgcn_raw_buffer_load_b16(r, (int)vb, (int)sb, 0);
   }
   __device__ __forceinline__ void put(__half v, unsigned vb, unsigned sb) const {
     __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(v), r, (int)vb, (int)sb, 0);
@@ -470,14 +478,28 @@ __device__ __forceinline__ void store_cell(Store* f, long long c, const KParams&
 // non-equilibrium storage.  Byte offsets: vb = cell index * element size,
 // population ip at ip * Nb (Nb = N * element size; NeqHalf: the FP16 part
 // follows the 16 N bytes of rho, u).
+// FP16 values are held zero-extended in 32-bit registers: as 16-bit values
+// the compiler packs two per register, which makes it wait for each pair of
+// loads as they are issued and defeats the prefetch.
+template <class Store>
+struct RawSlot {
+  using type = Store;
+};
+template <>
+struct RawSlot<__half> {
+  using type = unsigned;
+};
 template <class Real, class Store>
 struct RawCell {
-  Store v[Q];
+  typename RawSlot<Store>::type v[Q];
   __device__ __forceinline__ void load(const StateBuf<Store>& b, unsigned cell, unsigned N) {
     const unsigned vb = cell * (unsigned)sizeof(Store), Nb = N * (unsigned)sizeof(Store);
     static_for<Q>([&](auto I) {
       constexpr int ip = decltype(I)::value;
-      v[ip] = b.raw(vb, ip * Nb);
+      if constexpr (std::is_same<Store, __half>::value)
+        v[ip] = raw_b16_32(b.r, vb, ip * Nb);
+      else
+        v[ip] = b.raw(vb, ip * Nb);
     });
   }
   // f, rho and the momentum velocity u0 = j/rho
@@ -485,7 +507,7 @@ struct RawCell {
     static_for<Q>([&](auto I) {
       constexpr int ip = decltype(I)::value;
       if constexpr (std::is_same<Store, __half>::value)
-        ff[ip] = (Real)((float)wgt(ip) * (P.rho0f + __half2float(v[ip])));
+        ff[ip] = (Real)((float)wgt(ip) * (P.rho0f + __half2float(__ushort_as_half((unsigned short)v[ip]))));
       else
         ff[ip] = (Real)v[ip];
     });
@@ -500,14 +522,14 @@ struct RawCell {
 };
 template <class Real>
 struct RawCell<Real, NeqHalf> {
-  __half v[Q];
+  unsigned v[Q];  // FP16 bits, zero-extended (see RawSlot)
   float m[4];
   __device__ __forceinline__ void load(const StateBuf<NeqHalf>& b, unsigned cell, unsigned N) {
     const unsigned vf = cell * 4u, vh = cell * 2u;
     for (int q = 0; q < 4; q++) m[q] = b.rawf(vf, q * 4u * N);
     static_for<Q>([&](auto I) {
       constexpr int ip = decltype(I)::value;
-      v[ip] = b.raw(vh, 16u * N + ip * 2u * N);
+      v[ip] = b.raw32(vh, 16u * N + ip * 2u * N);
     });
   }
   __device__ __forceinline__ void decode(const KParams& P, Real (&ff)[Q], Real& rho, Real (&u0)[3]) const {
@@ -515,7 +537,12 @@ struct RawCell<Real, NeqHalf> {
     u0[0] = (Real)m[1];
     u0[1] = (Real)m[2];
     u0[2] = (Real)m[3];
-    neq_decode<Real>(rho, u0, v, ff);
+    __half s[Q];
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      s[ip] = __ushort_as_half((unsigned short)v[ip]);
+    });
+    neq_decode<Real>(rho, u0, s, ff);
   }
 };
 

@@ -130,12 +130,12 @@ int stats_axis(const Setup& s) {
   return pa < 0 ? 0 : pa;
 }
 
-void finish_statistics(const Setup& s, const double* sums, double* profiles, double diag[12]) {
+void finish_statistics(const Setup& s, const double* rows, const double* glob, double* profiles, double diag[12]) {
   const int pa = stats_axis(s);
   const int np = s.n[pa];
   const double cnt = (double)s.ncell() / np;
   for (int m = 0; m < np; m++) {
-    const double* a = sums + (size_t)m * 9;
+    const double* a = rows + (size_t)m * 9;
     double* o = profiles + (size_t)m * 9;
     for (int q = 0; q < 9; q++) o[q] = a[q] / cnt;
     o[6] -= o[0] * o[0];
@@ -145,7 +145,7 @@ void finish_statistics(const Setup& s, const double* sums, double* profiles, dou
     o[4] -= o[0] * o[1];
     o[5] -= o[1] * o[2];
   }
-  const double* g = sums + (size_t)np * 9;
+  const double* g = glob;
   for (int q = 0; q < 4; q++) {
     diag[q] = g[q] / (double)s.ncell();
     diag[4 + q] = g[4 + q];

@@ -28,10 +28,11 @@ void write_outputs(const Setup& s, int step, const double* rho, const double* ux
 // Axis of the planar statistics: the wall axis, else x (statistics_serial
 // averages over the planes normal to x).
 int stats_axis(const Setup& s);
-// From the device sums [n][9] + [4] sums + [4] min + [4] max to the
-// reference's profile columns (means, stresses uw/uv/vw, variances) and
-// diag (means, min, max of rho, u, v, w).
-void finish_statistics(const Setup& s, const double* sums, double* profiles, double diag[12]);
+// From the device sums, rows [n][9] and glob [4] sums + [4] min + [4] max,
+// to the reference's profile columns (means, stresses uw/uv/vw, variances)
+// and diag (means, min, max of rho, u, v, w).  profiles may be rows and diag
+// may be glob (in place).
+void finish_statistics(const Setup& s, const double* rows, const double* glob, double* profiles, double diag[12]);
 
 // VELMEAN.dat from profiles-2.dat (examples/channel/ProcessProfiles.py:17-79).
 // DHIT statistics files of stat_te (src/fortran/dhit/src/stat.f90:58-198),

@@ -300,9 +300,8 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
   }
 
   // f of the own column one plane ahead of its f-bar: the loads are issued
-  // one iteration before their f-bar needs them (FP16 deviation storage gains
-  // nothing from it and loads at use)
-  constexpr bool kPrefetch = !std::is_same<Store, __half>::value;
+  // one iteration before their f-bar needs them
+  constexpr bool kPrefetch = true;
   RawCell<Real, Store> fpre;
   if (kPrefetch && own) load_own((z0 + 1) % nz, fpre);
   // compact adjoint-face records with FP32 storage: rho and j of every cell
@@ -318,7 +317,12 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
     mb.st(jm[1], vb, 2 * pop);
     mb.st(jm[2], vb, 3 * pop);
   };
-  for (int p = z0; p < zend; p++) {
+  // One plane p: fbc / gc / lamc hold f-bar+, 4/3 f-bar+ - 1/3 f and lam' of
+  // plane p, and the plane computes those of p + 1 into fbn / gn / lamn.  The
+  // loop below runs the planes in pairs with the two register sets swapped,
+  // so no copy moves plane p+1's values into plane p's registers.
+  auto plane = [&](int p, Real (&fbc)[Q], Real (&fbn)[Q], Real (&gc)[Q], Real (&gn)[Q], Real (&lamc)[Q],
+                   Real (&lamn)[Q]) {
     const int kn = (p + 1) % nz;
     if constexpr (ADJ) load_lam(kn, lamn);
     if (own) {
@@ -540,15 +544,13 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
         constexpr int k = decltype(K)::value;
         fbm[k] = fbc[ZP[k]];
       });
-      static_for<Q>([&](auto I) {
-        constexpr int ip = decltype(I)::value;
-        fbc[ip] = fbn[ip];
-        if constexpr (!ADJ) gc[ip] = gn[ip];
-        if constexpr (ADJ && ip != 0) lamc[ip] = lamn[ip];
-      });
     }
     // no barrier: the next phase 1 writes A/An, which phase 3 does not read,
     // and the next phase 2 writes Cx/Dy only after the next phase-1 barrier
+  };
+  for (int p = z0; p < zend; p += 2) {
+    plane(p, fbc, fbn, gc, gn, lamc, lamn);
+    if (p + 1 < zend) plane(p + 1, fbn, fbc, gn, gc, lamn, lamc);
   }
 
   if (!ADJ && partial) {

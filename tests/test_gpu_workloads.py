@@ -156,3 +156,50 @@ def test_large_channel_256x192x256():
         err = _field_errors(out["fp64"], out[name])
         bad = {k: v for k, v in err.items() if not v <= FIELD_TOL}
         assert not bad, (name, err)
+
+
+def test_adjoint_channel_64x64x32_gradient():
+    # the adjoint workload at full size: transitional channel 64 x 64 x 32
+    # (seeded perturbed laminar state, Re = 2000), J = time-mean wall shear
+    # stress over T = 400 steps, control G ~ N(0, 1e-6) per cell, sqrt(T)
+    # checkpoints.  The FP64 gradient against central differences of J on
+    # 32 seeded random directions (bar 1e-4); the FP32 gradient against the
+    # FP64 one and against the same differences.
+    torch = pytest.importorskip("torch")
+    T, ndirs = 400, 32
+    dev = torch.device("cuda", 0)
+    grads, Js = {}, {}
+    fds = []
+    for name, prec in (("fp64", PREC_FP64), ("fp32", PREC_FP32)):
+        eng = _engine(configs.adjoint_channel(precision=name), prec)
+        nx, ny, nz = eng.shape
+        f0 = torch.empty((19, nz, ny, nx), dtype=torch.float64, device=dev)
+        eng.get_f_device(f0)
+        G = torch.from_numpy(1e-6 * np.random.default_rng(2024).standard_normal((3, nz, ny, nx))).to(dev)
+        gd = torch.empty_like(G)
+        eng.set_f_device(f0)
+        eng.set_cell_force_device(G)
+        Js[name], _ = eng.forward_adjoint(T, gd)
+        grads[name] = gd.cpu().numpy()
+        if name == "fp64":
+            rng = np.random.default_rng(7)
+
+            def run_J(ctrl):
+                eng.set_f_device(f0)
+                eng.set_cell_force_device(ctrl)
+                return eng.forward_record(T)
+
+            for _ in range(ndirs):
+                v = 1e-6 * rng.standard_normal(G.shape)
+                vd = torch.from_numpy(v).to(dev)
+                fd = (run_J(G + vd) - run_J(G - vd)) / 2.0
+                fds.append((fd, v))
+            eng.release_sweep()
+        eng.close()
+    g64, g32 = grads["fp64"], grads["fp32"]
+    err64 = [abs(fd - float((g64 * v).sum())) / abs(fd) for fd, v in fds]
+    err32 = [abs(fd - float((g32 * v).sum())) / abs(fd) for fd, v in fds]
+    assert max(err64) <= 1e-4, max(err64)
+    assert max(err32) <= 1e-4, max(err32)
+    assert np.abs(g32 - g64).max() <= 1e-4 * np.abs(g64).max()
+    assert abs(Js["fp32"] - Js["fp64"]) <= 1e-5 * abs(Js["fp64"])
