@@ -324,13 +324,24 @@ def study_tgv(dugks_amd, configs, torch, np, steps, every=10):
 def study_channel(dugks_amd, configs, torch, np, steps):
     """Re_tau = 180 channel 128^3 (tanh-stretched y, constant flow rate,
     synthetic log-law + Fourier state): mean-profile, rms and Reynolds-stress
-    errors of FP32 and FP16-store against FP64 after `steps` steps."""
+    errors of FP32, FP16 non-equilibrium and FP16 deviation storage against
+    FP64 after `steps` steps."""
     stats, rates = {}, {}
-    for m in ("fp64", "fp32", "fp16n", "fp16"):
-        prec = {"fp64": dugks_amd.PREC_FP64, "fp32": dugks_amd.PREC_FP32, "fp16": dugks_amd.PREC_FP16S,
-                "fp16n": dugks_amd.PREC_FP16N}[m]
-        eng = dugks_amd.Engine(configs.turbulent_channel(precision=m, nsteps=steps), precision=prec, device=0,
-                               stream=torch.cuda.current_stream().cuda_stream)
+    for m in ("fp64", "fp64_twin", "fp32", "fp16n", "fp16"):
+        prec = {"fp64": dugks_amd.PREC_FP64, "fp64_twin": dugks_amd.PREC_FP64, "fp32": dugks_amd.PREC_FP32,
+                "fp16": dugks_amd.PREC_FP16S, "fp16n": dugks_amd.PREC_FP16N}[m]
+        eng = dugks_amd.Engine(configs.turbulent_channel(precision=m.split("_")[0], nsteps=steps), precision=prec,
+                               device=0, stream=torch.cuda.current_stream().cuda_stream)
+        if m == "fp64_twin":
+            # the FP64 run from the initial state perturbed at the size of an
+            # FP32 rounding (3e-8 relative, seeded): how far the flow itself
+            # carries a rounding-sized difference over the run
+            f0 = torch.empty(19 * eng.ncell, dtype=torch.float64, device="cuda")
+            eng.get_f_device(f0)
+            gen = torch.Generator(device="cuda")
+            gen.manual_seed(20261203)
+            f0 *= 1 + 3e-8 * torch.randn(f0.shape, generator=gen, dtype=torch.float64, device="cuda")
+            eng.set_f_device(f0)
         t0 = time.perf_counter()
         done = 0
         while done < steps:
@@ -344,7 +355,7 @@ def study_channel(dugks_amd, configs, torch, np, steps):
         eng.close()
     out = {"workload": "Re_tau=180 channel 128^3, tanh y (alpha 0.85), constant flow rate", "steps": steps,
            "against": "fp64", "mlups": {k: round(v, 1) for k, v in rates.items()}}
-    for m in ("fp32", "fp16n", "fp16"):
+    for m in ("fp64_twin", "fp32", "fp16n", "fp16"):
         out[m] = {k: float(f"{v:.3e}") for k, v in field_errors(stats["fp64"], stats[m], np).items()}
     return out
 
@@ -437,8 +448,9 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=1000,
                     help="steps of the forward+adjoint sweep on the large channel (0: skip)")
     ap.add_argument("--study", choices=["tgv", "channel", "adjoint"],
-                    help="instead of the benchmark: the FP32 / FP16-store error study on the "
-                         "Taylor-Green vortex or the 128^3 channel (prints one JSON line)")
+                    help="instead of the benchmark: the reduced-precision error study on the "
+                         "Taylor-Green vortex or the 128^3 channel, or the adjoint gradient study "
+                         "(prints one JSON line)")
     ap.add_argument("--study-steps", type=int, default=0, help="steps of the study (default 1000 / 10000)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB read + 1 GiB write elementwise kernel 5 times "

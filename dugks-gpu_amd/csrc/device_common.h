@@ -62,9 +62,10 @@ struct Codec<__half> {
 };
 
 // FP16 non-equilibrium storage (the element tag of that mode's state arrays).
-// A state of N cells is one buffer: [4][N] float rho, u_x, u_y, u_z (u = j/rho,
-// the momentum velocity of f), then [Q][N] half s_i = n_i kNeqScale / w_i
-// with n_i = f_i - feq_i(rho, u).  The non-equilibrium part is small against
+// A state of N cells is one buffer: [4][N] float rho - 1, u_x, u_y, u_z
+// (u = j/rho, the momentum velocity of f; the density as its difference from
+// 1 keeps FP32 digits for the small density variations), then [Q][N] half
+// s_i = n_i kNeqScale / w_i with n_i = f_i - feq_i(rho, u).  The non-equilibrium part is small against
 // w_i, so FP16 rounds it far less than it would round f or f - w_i rho0; the
 // scale keeps typical |n_i| / w_i (1e-6 .. 1e-2) in FP16's normal range.
 struct NeqHalf {
@@ -197,6 +198,10 @@ __device__ __forceinline__ void st_state(const StateBuf<Store>& b, unsigned vb, 
     b.put((Store)v, vb, sb);
 }
 
+// a * b + c rounded once
+__device__ __forceinline__ float fma_rn(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_rn(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // Sum over the 64 lanes of a wave by an xor butterfly: a fixed order, and
 // every lane ends with the same bits (each level adds the same two values).
 __device__ __forceinline__ double wave_sum(double v) {
@@ -325,6 +330,22 @@ struct EqTerm {
       return (cF * B[wclass(ip)] + A[wclass(ip)]) * e;
     }
   }
+  // f - b f + G_ip given nb = -b, with f - b f formed by one fused
+  // multiply-add: the populations then sum to (1 - b) rho + sum G exactly up
+  // to unbiased rounding, whereas b' f with b' = 1 - b rounded to the
+  // arithmetic type would scale the mass by the rounding of b' on every
+  // application (a steady drift in FP32)
+  template <int ip>
+  __device__ __forceinline__ Real relax_keep(Real nb, Real f, Real cF) const {
+    const Real kept = fma_rn(nb, f, f);
+    if constexpr (ip == 0) {
+      return fma_rn(A[0], base, kept);
+    } else {
+      const Real cu = cdot<ip>(ux, uy, uz);
+      const Real e = cu * (cu * Real(4.5) + Real(3)) + base;
+      return fma_rn(cF * B[wclass(ip)] + A[wclass(ip)], e, kept);
+    }
+  }
   // s f + G_ip
   template <int ip>
   __device__ __forceinline__ Real relax(Real s, Real f, Real cF) const {
@@ -338,20 +359,99 @@ struct EqTerm {
   }
 };
 
+// ---- shifted populations (FP32 arithmetic of the fused step) ----
+// The fused FP32 step carries h_i = f_i - w_i (the populations less the rest
+// state of density 1) instead of f_i.  Its linear parts (the reconstruction,
+// the update, the wall extrapolation and bounce-back) have weights that sum
+// to one or take differences, so they act on h as on f, and only the
+// equilibrium terms change form.  |h_i| is a few tenths of w_i, so every
+// rounding is that much smaller than on f, and the rest state drops out of
+// the fluxes exactly instead of to rounding: in FP32 the accumulated
+// rounding of 10^4 steps otherwise reaches 1e-3 of the rms velocities.
+//
+// The relaxation f' = (1 - b) f + G_i, G_i = w_i rho e_i (b + K (c_i - u).F),
+// e_i = 1 + eps_i, eps_i = 3 c.u + 4.5 (c.u)^2 - 1.5 u.u, K = 3 Kf, in shifted
+// form (drho = rho - 1):
+//   h' = (1 - b) h + C0 + B c.F + (P + B c.F) eps_i,
+//   C0 = w (b drho - K rho u.F),  P = w rho (b - K u.F),  B = w K rho.
+template <class Real>
+struct EqShift {
+  Real ux, uy, uz, nuu, C0[3], P[3], B[3];
+  __device__ __forceinline__ EqShift(Real drho, Real ux_, Real uy_, Real uz_, const Real F[3], Real b, Real Kf)
+      : ux(ux_), uy(uy_), uz(uz_) {
+    const Real rho = Real(1) + drho;
+    nuu = Real(-1.5) * (ux * ux + uy * uy + uz * uz);
+    const Real uF = ux * F[0] + uy * F[1] + uz * F[2];
+    const Real K3 = Real(3) * Kf;
+    const Real c0 = b * drho - K3 * rho * uF;
+    const Real p0 = rho * (b - K3 * uF);
+    const Real b0 = K3 * rho;
+    for (int k = 0; k < 3; k++) {
+      const Real w = Real(wclass_weight(k));
+      C0[k] = w * c0;
+      P[k] = w * p0;
+      B[k] = w * b0;
+    }
+  }
+  // (1 - b) h + (G_ip - b w) given nb = -b, with h - b h one fused multiply-add
+  template <int ip>
+  __device__ __forceinline__ Real relax(Real nb, Real h, Real cF) const {
+    const Real kept = fma_rn(nb, h, h);
+    if constexpr (ip == 0) {
+      return fma_rn(P[0], nuu, C0[0] + kept);
+    } else {
+      constexpr int k = wclass(ip);
+      const Real cu = cdot<ip>(ux, uy, uz);
+      const Real eps = cu * (cu * Real(4.5) + Real(3)) + nuu;
+      return fma_rn(fma_rn(cF, B[k], P[k]), eps, fma_rn(cF, B[k], C0[k] + kept));
+    }
+  }
+};
+
+// Centre stage in shifted populations: hb_i = f-bar+_i - w_i from h_i, the
+// cell's drho = rho - 1 and momentum velocity u0 (with FP16 non-equilibrium
+// storage the stored ones, which the decoded populations match to the FP16
+// rounding of their non-equilibrium parts).
+template <class Real>
+__device__ __forceinline__ void centre_fbar_shift(const Real (&h)[Q], Real drho, const Real (&u0)[3],
+                                                  const Real F[3], Real hdt, Real bw0, Real bw2, Real (&hb)[Q]) {
+  const EqShift<Real> eq(drho, u0[0] + hdt * F[0], u0[1] + hdt * F[1], u0[2] + hdt * F[2], F, bw2, bw0);
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    hb[ip] = eq.template relax<ip>(-bw2, h[ip], cdot<ip>(F[0], F[1], F[2]));
+  });
+}
+
 // Centre-stage f-bar+ of one cell (calc_fb1 of the reference, src/dugks.cu:1131-1196):
 //   rho = sum f, u = j/rho + h F, feq_i(rho, u),
 //   fb_i = bw1 f_i + (bw2 + bw0 (c_i - u).F / RT) feq_i.
 // This form takes rho and the momentum velocity u0 = j/rho of the cell (from
 // cell_moments and moment_velocity below, or from FP16 non-equilibrium
 // storage, which holds them).
+// The stage conserves the cell's mass: sum fb = rho.  In FP32 the weights
+// and the rounded coefficients bias that sum by ~1e-8 of rho per step
+// (times bw2, which exceeds 1 when dt >> tau), a steady drift of the total
+// mass; there the rest population takes what the moving ones leave of rho
+// (the same value in exact arithmetic; rho must then be the sum of f), so
+// the stage keeps the mass to one unbiased rounding.  bw1 = 1 - bw2 enters
+// as f - bw2 f (EqTerm::relax_keep).
 template <class Real>
 __device__ __forceinline__ void centre_fbar(const Real (&f)[Q], Real rho, const Real (&u0)[3], const Real F[3],
-                                            Real hdt, Real bw0, Real bw1, Real bw2, Real (&fb)[Q]) {
+                                            Real hdt, Real bw0, Real bw2, Real (&fb)[Q]) {
   const EqTerm<Real> eq(rho, u0[0] + hdt * F[0], u0[1] + hdt * F[1], u0[2] + hdt * F[2], F, bw2, bw0);
   static_for<Q>([&](auto I) {
     constexpr int ip = decltype(I)::value;
-    fb[ip] = eq.template relax<ip>(bw1, f[ip], cdot<ip>(F[0], F[1], F[2]));
+    if constexpr (ip != 0 || sizeof(Real) != 4)
+      fb[ip] = eq.template relax_keep<ip>(-bw2, f[ip], cdot<ip>(F[0], F[1], F[2]));
   });
+  if constexpr (sizeof(Real) == 4) {
+    Real moving = fb[1];
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      if constexpr (ip >= 2) moving += fb[ip];
+    });
+    fb[0] = rho - moving;
+  }
 }
 
 // u0 = j / rho
@@ -377,13 +477,47 @@ __device__ __forceinline__ void cell_moments(const Real (&ff)[Q], Real& rho, Rea
   });
 }
 
+// rho and the momentum velocity u0 = j / rho of one cell; with `shifted`
+// the populations are h = f - w and rho_out is sum h = rho - 1.  FP32
+// populations are summed in FP64 and the results rounded to FP32 once: the
+// rounding of FP32 sums (about 1e-8 of j, a quarter of the velocity a
+// typical body force adds per step) repeats from step to step in a slowly
+// evolving flow and acts as a steady source, which over 10^4 steps of the
+// Re_tau = 180 channel moves the rms-velocity profiles by 1e-3 of their
+// size (FP64 sums: 2e-4).
 template <class Real>
-__device__ __forceinline__ void centre_fbar(const Real (&f)[Q], const Real F[3], Real hdt, Real bw0,
-                                            Real bw1, Real bw2, Real (&fb)[Q]) {
-  Real rho, j[3], u0[3];
-  cell_moments<Real>(f, rho, j);
-  moment_velocity<Real>(rho, j, u0);
-  centre_fbar<Real>(f, rho, u0, F, hdt, bw0, bw1, bw2, fb);
+__device__ __forceinline__ void cell_rho_u(const Real (&ff)[Q], Real& rho_out, Real (&u0)[3], bool shifted = false) {
+  if constexpr (sizeof(Real) == 4) {
+    double r, j[3];
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      const double v = (double)ff[ip];
+      if constexpr (ip == 0) r = v; else r += v;
+      if constexpr (ip == 1) j[0] = v; else if constexpr (CX[ip] != 0) j[0] += CX[ip] > 0 ? v : -v;
+      if constexpr (ip == 3) j[1] = v; else if constexpr (CY[ip] != 0) j[1] += CY[ip] > 0 ? v : -v;
+      if constexpr (ip == 5) j[2] = v; else if constexpr (CZ[ip] != 0) j[2] += CZ[ip] > 0 ? v : -v;
+    });
+    rho_out = (Real)r;
+    const double rho = shifted ? 1.0 + r : r;
+    // 1/rho: the hardware reciprocal and one Newton step (error ~1e-14)
+    double x = __builtin_amdgcn_rcp(rho);
+    x = __builtin_fma(x, __builtin_fma(-rho, x, 1.0), x);
+    u0[0] = (Real)(j[0] * x);
+    u0[1] = (Real)(j[1] * x);
+    u0[2] = (Real)(j[2] * x);
+  } else {
+    Real j[3];
+    cell_moments<Real>(ff, rho_out, j);
+    moment_velocity<Real>(shifted ? Real(1) + rho_out : rho_out, j, u0);
+  }
+}
+
+template <class Real>
+__device__ __forceinline__ void centre_fbar(const Real (&f)[Q], const Real F[3], Real hdt, Real bw0, Real bw2,
+                                            Real (&fb)[Q]) {
+  Real rho, u0[3];
+  cell_rho_u<Real>(f, rho, u0);
+  centre_fbar<Real>(f, rho, u0, F, hdt, bw0, bw2, fb);
 }
 
 // ---- FP16 non-equilibrium storage (NeqHalf) ----
@@ -415,9 +549,7 @@ __device__ __forceinline__ void neq_decode(Real rho, const Real (&u)[3], const _
 // rho, u = j/rho of f and s_i = (f_i - w_i rho e_i(u)) kNeqScale / w_i in FP16
 template <class Real>
 __device__ __forceinline__ void neq_encode(const Real (&ff)[Q], Real& rho, Real (&u)[3], __half (&s)[Q]) {
-  Real j[3];
-  cell_moments<Real>(ff, rho, j);
-  moment_velocity<Real>(rho, j, u);
+  cell_rho_u<Real>(ff, rho, u);
   const Real base = Real(1) - Real(1.5) * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
   const Real rs = rho * Real(kNeqScale);
   static_for<Q>([&](auto I) {
@@ -439,7 +571,7 @@ __device__ __forceinline__ void load_cell(const Store* f, long long c, const KPa
       sv[ip] = s[ip * P.N + c];
     });
     const Real u[3] = {(Real)m[P.N + c], (Real)m[2 * P.N + c], (Real)m[3 * P.N + c]};
-    neq_decode<Real>((Real)m[c], u, sv, ff);
+    neq_decode<Real>(Real(1) + (Real)m[c], u, sv, ff);
   } else {
     static_for<Q>([&](auto I) {
       constexpr int ip = decltype(I)::value;
@@ -457,7 +589,7 @@ __device__ __forceinline__ void store_cell(Store* f, long long c, const KParams&
     Real rho, u[3];
     __half sv[Q];
     neq_encode<Real>(ff, rho, u, sv);
-    m[c] = (float)rho;
+    m[c] = (float)(rho - Real(1));
     m[P.N + c] = (float)u[0];
     m[2 * P.N + c] = (float)u[1];
     m[3 * P.N + c] = (float)u[2];
@@ -502,6 +634,19 @@ struct RawCell {
         v[ip] = b.raw(vb, ip * Nb);
     });
   }
+  // Waits here for the loads of load() (an empty asm that reads and
+  // "writes" each slot): later reads of the slots then wait for nothing.
+  // The vector-memory counter counts loads and stores together, in issue
+  // order, so a wait for a load issued before a run of stores that the
+  // compiler cannot see on every path (stores under a branch) is compiled as
+  // a wait for those stores too; settling the slots before such stores
+  // avoids it.
+  __device__ __forceinline__ void settle() {
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      asm volatile("" : "+v"(v[ip]));
+    });
+  }
   // f, rho and the momentum velocity u0 = j/rho
   __device__ __forceinline__ void decode(const KParams& P, Real (&ff)[Q], Real& rho, Real (&u0)[3]) const {
     static_for<Q>([&](auto I) {
@@ -511,9 +656,20 @@ struct RawCell {
       else
         ff[ip] = (Real)v[ip];
     });
-    Real j[3];
-    cell_moments<Real>(ff, rho, j);
-    moment_velocity<Real>(rho, j, u0);
+    cell_rho_u<Real>(ff, rho, u0);
+  }
+  // shifted populations h = f - w (see EqShift), drho = sum h and the
+  // momentum velocity u0 = j / (1 + drho); f - w is exact for FP32 f within
+  // a factor 2 of w
+  __device__ __forceinline__ void decode_shift(const KParams& P, Real (&h)[Q], Real& drho, Real (&u0)[3]) const {
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      if constexpr (std::is_same<Store, __half>::value)
+        h[ip] = Real(wgt(ip)) * ((Real)__half2float(__ushort_as_half((unsigned short)v[ip])) + (P.rho0f - 1.0f));
+      else
+        h[ip] = (Real)v[ip] - Real(wgt(ip));
+    });
+    cell_rho_u<Real>(h, drho, u0, true);
   }
   // the rho and j only (FP32 storage path of the adjoint-face pass)
   __device__ __forceinline__ void moments(const Real (&ff)[Q], Real& rho, Real (&j)[3]) const {
@@ -532,8 +688,16 @@ struct RawCell<Real, NeqHalf> {
       v[ip] = b.raw32(vh, 16u * N + ip * 2u * N);
     });
   }
+  // as RawCell::settle
+  __device__ __forceinline__ void settle() {
+    for (int q = 0; q < 4; q++) asm volatile("" : "+v"(m[q]));
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      asm volatile("" : "+v"(v[ip]));
+    });
+  }
   __device__ __forceinline__ void decode(const KParams& P, Real (&ff)[Q], Real& rho, Real (&u0)[3]) const {
-    rho = (Real)m[0];
+    rho = Real(1) + (Real)m[0];
     u0[0] = (Real)m[1];
     u0[1] = (Real)m[2];
     u0[2] = (Real)m[3];
@@ -544,7 +708,63 @@ struct RawCell<Real, NeqHalf> {
     });
     neq_decode<Real>(rho, u0, s, ff);
   }
+  // shifted populations h_i = f_i - w_i = w_i (drho + rho eps_i + s_i / kNeqScale)
+  // straight from the stored drho, u and s (see EqShift)
+  __device__ __forceinline__ void decode_shift(const KParams& P, Real (&h)[Q], Real& drho, Real (&u0)[3]) const {
+    drho = (Real)m[0];
+    u0[0] = (Real)m[1];
+    u0[1] = (Real)m[2];
+    u0[2] = (Real)m[3];
+    const Real rho = Real(1) + drho;
+    const Real nuu = Real(-1.5) * (u0[0] * u0[0] + u0[1] * u0[1] + u0[2] * u0[2]);
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      Real eps = nuu;
+      if constexpr (ip != 0) {
+        const Real cu = cdot<ip>(u0[0], u0[1], u0[2]);
+        eps = cu * (cu * Real(4.5) + Real(3)) + nuu;
+      }
+      const Real sv = (Real)__half2float(__ushort_as_half((unsigned short)v[ip]));
+      h[ip] = Real(wgt(ip)) * fma_rn(sv, Real(1.0 / (double)kNeqScale), fma_rn(rho, eps, drho));
+    });
+  }
 };
+
+// Stores population ip of a cell given in shifted form h = f - w (FP32 f =
+// h + w, or the FP16 deviation h / w + 1 - rho0); not FP16 non-equilibrium
+// storage, which encodes whole cells (store_neq_shift).
+template <class Real, class Store>
+__device__ __forceinline__ void st_state_shift(const StateBuf<Store>& b, unsigned vb, unsigned sb, Real h,
+                                               const KParams& P, int ip) {
+  if constexpr (std::is_same<Store, __half>::value)
+    b.put(__float2half((float)h * (float)(1.0 / wgt(ip)) + (1.0f - P.rho0f)), vb, sb);
+  else
+    b.put((Store)(h + Real(wgt(ip))), vb, sb);
+}
+
+// FP16 non-equilibrium encoding of a cell given in shifted form: drho = sum
+// h, u0 = j / (1 + drho), s_i = (h_i / w_i - drho - rho eps_i) kNeqScale.
+// Returns drho and u0.
+template <class Real>
+__device__ __forceinline__ void store_neq_shift(const StateBuf<NeqHalf>& b, unsigned cell, unsigned N,
+                                                const Real (&h)[Q], Real& drho, Real (&u0)[3]) {
+  cell_rho_u<Real>(h, drho, u0, true);
+  const Real rho = Real(1) + drho;
+  const Real nuu = Real(-1.5) * (u0[0] * u0[0] + u0[1] * u0[1] + u0[2] * u0[2]);
+  const unsigned vf = cell * 4u, vh = cell * 2u;
+  b.putf((float)drho, vf, 0);
+  for (int q = 0; q < 3; q++) b.putf((float)u0[q], vf, (q + 1) * 4u * N);
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    Real eps = nuu;
+    if constexpr (ip != 0) {
+      const Real cu = cdot<ip>(u0[0], u0[1], u0[2]);
+      eps = cu * (cu * Real(4.5) + Real(3)) + nuu;
+    }
+    const Real n = fma_rn(h[ip], Real(1.0 / wgt(ip)), -fma_rn(rho, eps, drho));
+    b.put(__float2half((float)(n * Real(kNeqScale))), vh, 16u * N + ip * 2u * N);
+  });
+}
 
 // Stores the new f of a cell through a StateBuf; returns its rho and momentum
 // velocity u0 (needed for the encoding, and by the flow-rate partials) when
@@ -557,7 +777,7 @@ __device__ __forceinline__ void store_raw_cell(const StateBuf<Store>& b, unsigne
     __half s[Q];
     neq_encode<Real>(ff, rho, u0, s);
     const unsigned vf = cell * 4u, vh = cell * 2u;
-    b.putf((float)rho, vf, 0);
+    b.putf((float)(rho - Real(1)), vf, 0);
     for (int q = 0; q < 3; q++) b.putf((float)u0[q], vf, (q + 1) * 4u * N);
     static_for<Q>([&](auto I) {
       constexpr int ip = decltype(I)::value;
@@ -569,11 +789,7 @@ __device__ __forceinline__ void store_raw_cell(const StateBuf<Store>& b, unsigne
       constexpr int ip = decltype(I)::value;
       st_state<Real, Store>(b, vb, ip * Nb, ff[ip], P, ip);
     });
-    if (want_u) {
-      Real j[3];
-      cell_moments<Real>(ff, rho, j);
-      moment_velocity<Real>(rho, j, u0);
-    }
+    if (want_u) cell_rho_u<Real>(ff, rho, u0);
   }
 }
 

@@ -16,9 +16,15 @@ __device__ constexpr int dirs(int k) {
   return A == 0 ? XD[k] : (A == 1 ? YD[k] : ZD[k]);
 }
 
+// (1 - m) a + m b, evaluated as a + m (b - a): a constant stays exactly
+// constant whatever the rounding of m.  In FP32, (1 - m) and m rounded apart
+// do not sum to 1, which on a stretched axis biases every face value by a
+// fixed fraction of its size; the flux divergence turns that into a steady
+// source that shifts the mean flow (the 10^4-step channel's FP32 profiles
+// drift by up to 3e-3 in the other form).
 template <class Real>
 __device__ __forceinline__ Real lerp_(Real a, Real b, Real m) {
-  return (Real(1) - m) * a + m * b;
+  return fma_rn(m, b - a, a);
 }
 
 template <class Real>
@@ -28,69 +34,94 @@ __device__ __forceinline__ void ambient_force(const KParams& P, Real F[3]) {
   F[2] = (Real)P.famb_dev[2];
 }
 
-// f-bar+ of interior cell (i, j, k) (0-based).
-template <class Real, class Store>
+// f-bar+ of interior cell (i, j, k) (0-based); SHIFT: in shifted form
+// f-bar+ - w (see EqShift).
+template <class Real, class Store, bool SHIFT>
 __device__ __forceinline__ void fbar_cell(const KParams& P, const StateBuf<Store>& f, unsigned Nb, int i, int j,
                                           int k, const Real Famb[3], Real (&fb)[Q]) {
   const unsigned c = ((unsigned)k * P.n[1] + j) * P.n[0] + i;
   RawCell<Real, Store> raw;
   raw.load(f, c, (unsigned)P.N);
   Real ff[Q], rho, u0[3];
-  raw.decode(P, ff, rho, u0);
+  if constexpr (SHIFT)
+    raw.decode_shift(P, ff, rho, u0);
+  else
+    raw.decode(P, ff, rho, u0);
   Real F[3] = {Famb[0], Famb[1], Famb[2]};
   if (P.fcell) {
     F[0] += (Real)P.fcell[c];
     F[1] += (Real)P.fcell[P.N + c];
     F[2] += (Real)P.fcell[2 * P.N + c];
   }
-  centre_fbar<Real>(ff, rho, u0, F, (Real)P.hdt, (Real)P.bw0, (Real)P.bw1, (Real)P.bw2, fb);
+  if constexpr (SHIFT)
+    centre_fbar_shift<Real>(ff, rho, u0, F, (Real)P.hdt, (Real)P.bw0, (Real)P.bw2, fb);
+  else
+    centre_fbar<Real>(ff, rho, u0, F, (Real)P.hdt, (Real)P.bw0, (Real)P.bw2, fb);
 }
 
 // f-bar+ of a column that may be a wall ghost in x and/or y: the ghost value
 // is the linear extrapolation q f(1) + (1-q) f(2) from the wall side.
 // gi, gj are 0-based; a wall ghost is -1 or n; periodic indices are wrapped
-// by the caller.
-template <class Real, class Store>
-__device__ void fbar_column(const KParams& P, const StateBuf<Store>& f, unsigned Nb, int gi, int gj, int k,
+// by the caller.  SHIFT: in shifted form (the extrapolation weights sum to 1).
+template <class Real, class Store, bool SHIFT>
+__device__ void fbar_column(const KParams& P, const StateBuf<Store>& f, unsigned Nb, int gi, int gj, int kz,
                             const Real Famb[3], Real (&fb)[Q]) {
   const bool gx = gi < 0 || gi >= P.n[0];
   const bool gy = gj < 0 || gj >= P.n[1];
   if (!gx && !gy) {
-    fbar_cell<Real, Store>(P, f, Nb, gi, gj, k, Famb, fb);
+    fbar_cell<Real, Store, SHIFT>(P, f, Nb, gi, gj, kz, Famb, fb);
     return;
   }
-  int xs[2] = {gi, gi}, ys[2] = {gj, gj};
-  Real wx[2] = {1, 0}, wy[2] = {1, 0};
+  // the two columns (and weights) of the extrapolation along each ghost axis;
+  // a non-ghost axis uses the column itself with weight 1
+  int x0 = gi, x1 = gi, y0 = gj, y1 = gj;
+  Real qx = Real(1), qy = Real(1);
   if (gx) {
     const bool lo = gi < 0;
-    const Real q = (Real)(lo ? P.qlo[0] : P.qhi[0]);
-    xs[0] = lo ? 0 : P.n[0] - 1;
-    xs[1] = lo ? 1 : P.n[0] - 2;
-    wx[0] = q;
-    wx[1] = Real(1) - q;
+    qx = (Real)(lo ? P.qlo[0] : P.qhi[0]);
+    x0 = lo ? 0 : P.n[0] - 1;
+    x1 = lo ? 1 : P.n[0] - 2;
   }
   if (gy) {
     const bool lo = gj < 0;
-    const Real q = (Real)(lo ? P.qlo[1] : P.qhi[1]);
-    ys[0] = lo ? 0 : P.n[1] - 1;
-    ys[1] = lo ? 1 : P.n[1] - 2;
-    wy[0] = q;
-    wy[1] = Real(1) - q;
+    qy = (Real)(lo ? P.qlo[1] : P.qhi[1]);
+    y0 = lo ? 0 : P.n[1] - 1;
+    y1 = lo ? 1 : P.n[1] - 2;
   }
-  // tensor-product extrapolation (the x -> y ghost fill of the reference
-  // gives the same linear combination)
-  bool first = true;
-  for (int b = 0; b < (gy ? 2 : 1); b++)
-    for (int a = 0; a < (gx ? 2 : 1); a++) {
-      const Real w = wx[a] * wy[b];
+  // The extrapolation q f(1) + (1 - q) f(2) along a ghost axis is taken as
+  // f(2) + q (f(1) - f(2)) (exact for constants, see lerp_); a corner ghost
+  // of two wall axes takes the tensor product of the two (the x -> y ghost
+  // fill of the reference gives the same linear combination).  f-bar+ has
+  // one code copy per case in a rolled loop, and no indexed local arrays
+  // (those would live in scratch memory): these lanes are few.
+  if (!(gx && gy)) {
+    const Real q = gx ? qx : qy;
+#pragma unroll 1
+    for (int k = 0; k < 2; k++) {
+      // k = 0: the second column (x1 or y1), k = 1: the first
+      const int ci = gx ? (k ? x0 : x1) : gi, cj = gy ? (k ? y0 : y1) : gj;
       Real t[Q];
-      fbar_cell<Real, Store>(P, f, Nb, xs[a], ys[b], k, Famb, t);
+      fbar_cell<Real, Store, SHIFT>(P, f, Nb, ci, cj, kz, Famb, t);
       static_for<Q>([&](auto I) {
         constexpr int ip = decltype(I)::value;
-        fb[ip] = first ? w * t[ip] : fb[ip] + w * t[ip];
+        fb[ip] = k ? fma_rn(q, t[ip] - fb[ip], fb[ip]) : t[ip];
       });
-      first = false;
     }
+    return;
+  }
+  bool first = true;
+#pragma unroll 1
+  for (int k = 0; k < 4; k++) {
+    const int a = k & 1, b = k >> 1;
+    const Real w = (a ? Real(1) - qx : qx) * (b ? Real(1) - qy : qy);
+    Real t[Q];
+    fbar_cell<Real, Store, SHIFT>(P, f, Nb, a ? x1 : x0, b ? y1 : y0, kz, Famb, t);
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      fb[ip] = first ? w * t[ip] : fb[ip] + w * t[ip];
+    });
+    first = false;
+  }
 }
 
 // Index of population ip in the crossing set of axis A (-1 if it does not
@@ -263,6 +294,20 @@ struct Face {
       constexpr int k = decltype(K)::value;
       constexpr int ip = dirs<A>(k);
       out[k] = eq.template relax<ip>(fw1, cross[k], cdot<ip>(F[0], F[1], F[2]));
+    });
+  }
+  // collide in shifted populations (the face holds f-bar - w, its rho the
+  // sum of those, drho): out - w for the crossing populations (see EqShift)
+  __device__ __forceinline__ void collide_shift(const Real F[3], Real hdt, Real fw0, Real fw2, Real (&out)[10]) {
+    moments();
+    const Real inv = rcp_real(Real(1) + rho);
+    const Real hh = Real(0.5) * hdt;
+    const EqShift<Real> eq(rho, j[0] * inv + hh * F[0], j[1] * inv + hh * F[1], j[2] * inv + hh * F[2], F, fw2,
+                           fw0);
+    static_for<10>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      constexpr int ip = dirs<A>(k);
+      out[k] = eq.template relax<ip>(-fw2, cross[k], cdot<ip>(F[0], F[1], F[2]));
     });
   }
   // The transposed collision up to its equilibrium coefficients: the

@@ -60,6 +60,12 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
                                                           double* __restrict__ partial, AdjFaceIO adj) {
   using T = Tile<TX, TY>;
   constexpr int AC = T::AC;
+  // With FP16 non-equilibrium storage the step carries shifted populations
+  // h = f - w (see EqShift): f-bar+, the LDS planes, the face values and the
+  // update are all f - w, decoded from and encoded to the stored rho - 1, u
+  // and non-equilibrium part without forming f (fewer operations than
+  // through f).  The other storage modes and the adjoint-face pass work on f.
+  constexpr bool kShift = std::is_same<Store, NeqHalf>::value && !ADJ;
   // LDS planes indexed (row, column).  16-wide rows put four rows in a wave:
   // column-major planes (column stride TY + 2, even) then keep the lanes of
   // a ds_read on distinct banks; 32-wide rows are conflict-free row-major.
@@ -199,7 +205,10 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
   auto zface = [&](Real (&out)[10]) {
     Face<Real, 2> fc;
     zface_rec(fc);
-    fc.collide(Famb, hdt, fw0, fw1, fw2, out);
+    if constexpr (kShift)
+      fc.collide_shift(Famb, hdt, fw0, fw2, out);
+    else
+      fc.collide(Famb, hdt, fw0, fw1, fw2, out);
   };
   // adjoint-face mode: lam' of population ip at an interior cell (0 for a
   // ghost, cell < 0) and the store of a face's cotangents
@@ -261,30 +270,45 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
   };
   // The update of plane p needs 4/3 f-bar+(p) - 1/3 f(p); it is formed when
   // f(p) arrives for f-bar+(p), so f is read once per step: gc holds it for
-  // the current plane, gn for the next.
+  // the current plane, gn for the next.  Written f-bar+ + (f-bar+ - f)/3:
+  // the two terms of the difference carry the same mass, so the rounding of
+  // 1/3 biases nothing (the rounded 4/3 and 1/3 of the plain form add about
+  // 3e-8 of the mass per step in FP32).
   Real gc[Q], gn[Q];
   auto form_g = [&](const Real (&fb)[Q], const Real (&ff)[Q], Real (&g)[Q]) {
     static_for<Q>([&](auto I) {
       constexpr int ip = decltype(I)::value;
-      g[ip] = fb[ip] * Real(4.0 / 3.0) - ff[ip] * Real(1.0 / 3.0);
+      g[ip] = fb[ip] + (fb[ip] - ff[ip]) * Real(1.0 / 3.0);
     });
   };
+
+  // f of the own column one plane ahead of its f-bar: the loads are issued
+  // one iteration before their f-bar needs them.  The first (plane z0+1) is
+  // issued ahead of the pre-loop's loads, so it arrives with them.
+  constexpr bool kPrefetch = true;
+  RawCell<Real, Store> fpre;
+  if (kPrefetch && own) load_own((z0 + 1) % nz, fpre);
 
   // ---- pre-loop: f-bar at z0-1 and z0, and the -z face of plane z0 ----
   {
     const int km = (z0 - 1 + nz) % nz;
     if (active) {
-      fbar_column<Real, Store>(P, bin, Nb, gi, gj, km, Famb, fbn);
+      fbar_column<Real, Store, kShift>(P, bin, Nb, gi, gj, km, Famb, fbn);
       if (!ADJ && own) {
         RawCell<Real, Store> raw;
         Real ff[Q], F[3], rho, u0[3];
         load_own(z0, raw);
-        raw.decode(P, ff, rho, u0);
         force_own(z0, F);
-        centre_fbar<Real>(ff, rho, u0, F, hdt, (Real)P.bw0, (Real)P.bw1, (Real)P.bw2, fbc);
+        if constexpr (kShift) {
+          raw.decode_shift(P, ff, rho, u0);
+          centre_fbar_shift<Real>(ff, rho, u0, F, hdt, (Real)P.bw0, (Real)P.bw2, fbc);
+        } else {
+          raw.decode(P, ff, rho, u0);
+          centre_fbar<Real>(ff, rho, u0, F, hdt, (Real)P.bw0, (Real)P.bw2, fbc);
+        }
         if constexpr (!ADJ) form_g(fbc, ff, gc);
       } else {
-        fbar_column<Real, Store>(P, bin, Nb, gi, gj, z0, Famb, fbc);
+        fbar_column<Real, Store, kShift>(P, bin, Nb, gi, gj, z0, Famb, fbc);
       }
       static_for<5>([&](auto K) {
         constexpr int k = decltype(K)::value;
@@ -299,11 +323,7 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
     __syncthreads();
   }
 
-  // f of the own column one plane ahead of its f-bar: the loads are issued
-  // one iteration before their f-bar needs them
-  constexpr bool kPrefetch = true;
-  RawCell<Real, Store> fpre;
-  if (kPrefetch && own) load_own((z0 + 1) % nz, fpre);
+  if constexpr (kPrefetch) fpre.settle();
   // compact adjoint-face records with FP32 storage: rho and j of every cell
   // for the gather, written by the interior lanes from the prefetched f.
   // A chunk writes its planes z0+1 .. zend (mod nz), so the chunks cover
@@ -319,8 +339,8 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
   };
   // One plane p: fbc / gc / lamc hold f-bar+, 4/3 f-bar+ - 1/3 f and lam' of
   // plane p, and the plane computes those of p + 1 into fbn / gn / lamn.  The
-  // loop below runs the planes in pairs with the two register sets swapped,
-  // so no copy moves plane p+1's values into plane p's registers.
+  // step's loop below runs the planes in pairs with the two register sets
+  // swapped, so no copy moves plane p+1's values into plane p's registers.
   auto plane = [&](int p, Real (&fbc)[Q], Real (&fbn)[Q], Real (&gc)[Q], Real (&gn)[Q], Real (&lamc)[Q],
                    Real (&lamn)[Q]) {
     const int kn = (p + 1) % nz;
@@ -330,21 +350,26 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
       Real F[3];
       force_own(kn, F);
       Real ff[Q], rho, u0[3];
-      fpre.decode(P, ff, rho, u0);
-      if constexpr (ADJ && COMPACT && std::is_same<Store, float>::value) {
-        if (interior) {
-          Real jm[3];
-          cell_moments<Real>(ff, rho, jm);
-          store_moments(kn, rho, jm);
+      if constexpr (kShift) {
+        fpre.decode_shift(P, ff, rho, u0);
+        centre_fbar_shift<Real>(ff, rho, u0, F, hdt, (Real)P.bw0, (Real)P.bw2, fbn);
+      } else {
+        fpre.decode(P, ff, rho, u0);
+        if constexpr (ADJ && COMPACT && std::is_same<Store, float>::value) {
+          if (interior) {
+            Real jm[3];
+            cell_moments<Real>(ff, rho, jm);
+            store_moments(kn, rho, jm);
+          }
         }
+        centre_fbar<Real>(ff, rho, u0, F, hdt, (Real)P.bw0, (Real)P.bw2, fbn);
       }
-      centre_fbar<Real>(ff, rho, u0, F, hdt, (Real)P.bw0, (Real)P.bw1, (Real)P.bw2, fbn);
       if constexpr (!ADJ) {
         if (interior) form_g(fbn, ff, gn);
       }
       if (kPrefetch && p + 1 < zend) load_own((p + 2) % nz, fpre);
     } else if (active) {
-      fbar_column<Real, Store>(P, bin, Nb, gi, gj, kn, Famb, fbn);
+      fbar_column<Real, Store, kShift>(P, bin, Nb, gi, gj, kn, Famb, fbn);
     }
     // ---- phase 1: Z(p) (tangential z-lerp of f-bar(p)) and Zn(p+1/2) ----
     if (active) {
@@ -406,7 +431,10 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
           constexpr int ip = XD[k];
           mu[k] = Real(CX[ip]) * (uR * Lam[ip][ia(r, c + 1)] - uL * Lam[ip][ia(r, c)]);
         });
-        const int slot = (gi == 0 && P.periodic[0]) ? nx : gi;
+        // (no "cond ? nx : gi": the optimiser turns a select between two
+        // captured variables into a select between their addresses, which
+        // keeps them in scratch memory)
+        const int slot = gi + ((gi == 0 && P.periodic[0]) ? nx : 0);
         const long long fs = ((long long)(p + 1) * e1 + (gj + 1)) * e0 + slot;
         if constexpr (COMPACT) {
           Real Ac, B[3], rec[kCompact];
@@ -419,7 +447,10 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
         }
       } else {
         Real out[10];
-        fc.collide(Famb, hdt, fw0, fw1, fw2, out);
+        if constexpr (kShift)
+          fc.collide_shift(Famb, hdt, fw0, fw2, out);
+        else
+          fc.collide(Famb, hdt, fw0, fw1, fw2, out);
         static_for<10>([&](auto K) {
           constexpr int k = decltype(K)::value;
           Cx[k][ic(r - 1, c)] = out[k];
@@ -455,7 +486,7 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
           constexpr int ip = YD[k];
           mu[k] = Real(CY[ip]) * (uR * Lam[ip][ia(r, c + 1)] - uL * Lam[ip][ia(r - 1, c + 1)]);
         });
-        const int slot = (gj == 0 && P.periodic[1]) ? ny : gj;
+        const int slot = gj + ((gj == 0 && P.periodic[1]) ? ny : 0);
         const long long fs = ((long long)(p + 1) * e1 + slot) * e0 + (gi + 1);
         if constexpr (COMPACT) {
           Real Ac, B[3], rec[kCompact];
@@ -468,7 +499,10 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
         }
       } else {
         Real out[10];
-        fc.collide(Famb, hdt, fw0, fw1, fw2, out);
+        if constexpr (kShift)
+          fc.collide_shift(Famb, hdt, fw0, fw2, out);
+        else
+          fc.collide(Famb, hdt, fw0, fw1, fw2, out);
         static_for<10>([&](auto K) {
           constexpr int k = decltype(K)::value;
           Dy[k][id(r - 1, c)] = out[k];
@@ -503,6 +537,10 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
     }
     __syncthreads();
     // ---- phase 3: update of plane p ----
+    // The f prefetched at the top of this plane arrived long ago: take it
+    // before this plane's stores, so that the next plane's decode does not
+    // wait for them (see RawCell::settle).
+    if constexpr (!ADJ && kPrefetch) fpre.settle();
     if constexpr (!ADJ) {
       if (interior) {
         const unsigned cell = ((unsigned)p * ny + gj) * nx + gi;
@@ -523,16 +561,21 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
           }
           // per-population stores go out as each value is ready; FP16
           // non-equilibrium storage encodes the whole cell below
-          if constexpr (!kNeq) st_state<Real, Store>(bout, vb, ip * Nb, v, P, ip);
+          if constexpr (!kNeq) {
+            if constexpr (kShift)
+              st_state_shift<Real, Store>(bout, vb, ip * Nb, v, P, ip);
+            else
+              st_state<Real, Store>(bout, vb, ip * Nb, v, P, ip);
+          }
           fnew[ip] = v;
         });
         Real rr, u0n[3];
-        if constexpr (kNeq) {
+        if constexpr (kNeq && kShift) {
+          store_neq_shift<Real>(bout, cell, (unsigned)P.N, fnew, rr, u0n);
+        } else if constexpr (kNeq) {
           store_raw_cell<Real, Store>(bout, cell, (unsigned)P.N, P, fnew, true, rr, u0n);
         } else if (partial) {
-          Real jj[3];
-          cell_moments<Real>(fnew, rr, jj);
-          moment_velocity<Real>(rr, jj, u0n);
+          cell_rho_u<Real>(fnew, rr, u0n, kShift);
         }
         // bulk velocity of the new state for the flow-rate controller
         // (per-cell moments in the arithmetic type, the sum over cells in FP64)
@@ -548,9 +591,27 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
     // no barrier: the next phase 1 writes A/An, which phase 3 does not read,
     // and the next phase 2 writes Cx/Dy only after the next phase-1 barrier
   };
-  for (int p = z0; p < zend; p += 2) {
-    plane(p, fbc, fbn, gc, gn, lamc, lamn);
-    if (p + 1 < zend) plane(p + 1, fbn, fbc, gn, gc, lamn, lamc);
+  // The step runs planes in pairs; the adjoint-face pass (more registers
+  // live across the loop) one plane per iteration with the values of p+1
+  // copied down.
+  constexpr bool kPairs = !ADJ;
+  if constexpr (kPairs) {
+    for (int p = z0; p < zend; p += 2) {
+      plane(p, fbc, fbn, gc, gn, lamc, lamn);
+      if (p + 1 < zend) plane(p + 1, fbn, fbc, gn, gc, lamn, lamc);
+    }
+  } else {
+    for (int p = z0; p < zend; p++) {
+      plane(p, fbc, fbn, gc, gn, lamc, lamn);
+      if (active) {
+        static_for<Q>([&](auto I) {
+          constexpr int ip = decltype(I)::value;
+          fbc[ip] = fbn[ip];
+          if constexpr (ADJ && ip != 0) lamc[ip] = lamn[ip];
+          if constexpr (!ADJ) gc[ip] = gn[ip];
+        });
+      }
+    }
   }
 
   if (!ADJ && partial) {

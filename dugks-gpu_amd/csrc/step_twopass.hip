@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(256) k_fbar(KParams P, const Store* __restrict
     F[1] += (Real)P.fcell[P.N + c];
     F[2] += (Real)P.fcell[2 * P.N + c];
   }
-  centre_fbar<Real>(ff, F, (Real)P.hdt, (Real)P.bw0, (Real)P.bw1, (Real)P.bw2, fb);
+  centre_fbar<Real>(ff, F, (Real)P.hdt, (Real)P.bw0, (Real)P.bw2, fb);
   static_for<Q>([&](auto I) {
     constexpr int ip = decltype(I)::value;
     fbx[ip * P.eN + e] = fb[ip];
@@ -77,15 +77,18 @@ __global__ void __launch_bounds__(256) k_ghost(KParams P, Real* __restrict__ fbx
       p[g0] = p[in];
       p[g1] = p[i1];
     } else {
-      p[g0] = qlo * p[i1] + (Real(1) - qlo) * p[i2];
-      p[g1] = qhi * p[in] + (Real(1) - qhi) * p[in1];
+      // q f(1) + (1 - q) f(2) as f(2) + q (f(1) - f(2)) (see lerp_)
+      p[g0] = fma_rn(qlo, p[i1] - p[i2], p[i2]);
+      p[g1] = fma_rn(qhi, p[in] - p[in1], p[in1]);
     }
   }
 }
 
+// (1 - m) a + m b as a + m (b - a): constants stay exactly constant whatever
+// the rounding of m (see the fused kernel's lerp_)
 template <class Real>
 __device__ __forceinline__ Real lerp_(Real a, Real b, Real m) {
-  return (Real(1) - m) * a + m * b;
+  return fma_rn(m, b - a, a);
 }
 
 // Face collision of the reference (f2ruvwh + feq_s, src/dugks.cu:675-709,
@@ -191,11 +194,13 @@ __global__ void __launch_bounds__(256) k_flux(KParams P, const Store* __restrict
   ambient(P, F);
   const Real hdt = (Real)P.hdt, fw0 = (Real)P.fw0, fw1 = (Real)P.fw1, fw2 = (Real)P.fw2;
   Real acc[Q];
-  // f~+ = 4/3 f-bar+ - 1/3 f~ (src/dugks.cu:954)
+  // f~+ = 4/3 f-bar+ - 1/3 f~ (src/dugks.cu:954), as f-bar+ + (f-bar+ - f~)/3
+  // (no mass bias from the rounding of 1/3)
   load_cell<Real, Store>(fin, c, P, acc);
   static_for<Q>([&](auto I) {
     constexpr int ip = decltype(I)::value;
-    acc[ip] = fbx[ip * P.eN + e] * Real(4.0 / 3.0) - acc[ip] * Real(1.0 / 3.0);
+    const Real fb = fbx[ip * P.eN + e];
+    acc[ip] = fb + (fb - acc[ip]) * Real(1.0 / 3.0);
   });
   auto face = [&](auto Ac, auto Sc) {
     constexpr int A = decltype(Ac)::value;

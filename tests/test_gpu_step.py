@@ -139,7 +139,8 @@ def test_cell_force_matches_oracle(rng, kernel):
 # each store.  FP16 non-equilibrium storage rounds only s = (f - feq(rho, u))
 # 1024 / w, rho and u stay FP32: round to nearest moves each population by at
 # most 2^-11 |f - feq| (half an FP16 ulp of s, relative), on top of the FP32
-# arithmetic; that bound is checked per population (tol = None)
+# arithmetic of the decode, the step and the encode (twice the FP32 step's
+# bar); that bound is checked per population (tol = None)
 @pytest.mark.parametrize("precision,tol", [(PREC_FP32, 2e-6), (PREC_FP16S, 1.5e-3), (PREC_FP16N, None)],
                          ids=["fp32", "fp16", "fp16n"])
 def test_reduced_precision_one_step(rng, precision, tol, kernel, nx):
@@ -158,7 +159,7 @@ def test_reduced_precision_one_step(rng, precision, tol, kernel, nx):
         return
     rho, j = moments(want)
     neq = want - feq(rho, j / rho[..., None])
-    bound = 2.0 ** -11 * np.abs(neq) + 2e-6 * np.abs(want).max()
+    bound = 2.0 ** -11 * np.abs(neq) + 4e-6 * np.abs(want).max()
     assert np.all(np.abs(got - want) <= bound), float((np.abs(got - want) / bound).max())
     # the bound is not slack: the FP16 rounding is visible above the FP32 part
     assert _relerr(got, want) > 2e-6
@@ -398,23 +399,32 @@ def test_mirror_equivariance(rng, case, kernel):
 
 @pytest.mark.parametrize("kernel", KERNELS, ids=["twopass", "fused"])
 def test_fp16n_storage_round_trip(rng, kernel):
-    # FP16 non-equilibrium storage keeps rho and u = j/rho in FP32: a state
-    # read back has the moments of the state written to FP32 rounding, and
-    # its populations within the FP16 rounding of f - feq
+    # FP16 non-equilibrium storage keeps rho and u = j/rho in FP32 and rounds
+    # s = (f - feq(rho, u)) 1024 / w to FP16: a state read back is within
+    # half an FP16 ulp of s (2^-11 |f - feq|, relative) of the state written,
+    # plus FP32 terms, population by population.  The rounding does not
+    # preserve moments, so rho and j read back are within the sum of those
+    # bounds, not FP32-exact; storing the read-back state again moves it by
+    # at most the same rounding (the codec is not idempotent: rho and u of
+    # the read-back state differ from the stored ones).
     text = configs.base(nx=32, ny=12, nz=4, periods_y=0, visc=0.02, ambient_fx=2e-4,
                         xwidth=3.2, ywidth=1.2, zwidth=0.4)
     eng = _engine(text, precision=PREC_FP16N, kernel=kernel)
+
+    def bound(f):
+        rho, j = moments(f)
+        return np.abs(f - feq(rho, j / rho[..., None])) * 2.0 ** -11 + 1e-6 * W
+
     f0 = random_state(eng.shape, rng)
     eng.set_f(f0)
     f1 = eng.get_f()
+    b0 = bound(f0)
+    assert np.all(np.abs(f1 - f0) <= b0)
     r0, j0 = moments(f0)
     r1, j1 = moments(f1)
-    neq = f0 - feq(r0, j0 / r0[..., None])
-    assert np.abs(r1 - r0).max() <= 1e-6 * np.abs(r0).max()
-    assert np.abs(j1 - j0).max() <= 1e-6 * np.abs(r0).max()
-    err = np.abs(f1 - f0) / W
-    # half ulp of FP16 relative to |neq| / w, plus the FP32 terms
-    assert np.all(err <= np.abs(neq) / W * 2.0 ** -11 + 1e-6)
-    # storing the read-back state again changes no bit
+    assert np.all(np.abs(r1 - r0) <= b0.sum(-1))
+    assert np.all(np.abs(j1 - j0) <= np.einsum("...q,qa->...a", b0, np.abs(C)))
+    # the rounding is visible: FP32 storage would keep rho to ~1e-7
+    assert np.abs(r1 - r0).max() > 1e-6
     eng.set_f(f1)
-    assert np.array_equal(eng.get_f(), f1)
+    assert np.all(np.abs(eng.get_f() - f1) <= bound(f1))
