@@ -123,6 +123,11 @@ int create(const Setup& s, const dugks_opts* opts, dugks_ctx** out) {
     hipGetDevice(&c.device);
   }
   hipSetDevice(c.device);
+  // address of this device's statistics scratch (dugks_get_statistics_noalloc)
+  if (noalloc_stats_scratch(c.device, &c.stats_noalloc) != hipSuccess) {
+    g_init_err = "cannot resolve the statistics scratch on this device";
+    return DUGKS_EHIP;
+  }
   if (o.stream) {
     c.stream = static_cast<hipStream_t>(o.stream);
   } else {
@@ -513,6 +518,30 @@ int dugks_get_statistics(dugks_ctx* ctx, double* profiles, double diag[12], int*
   HIPCHK(hipMemcpyAsync(sums.data(), out, sums.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   HIPCHK(hipStreamSynchronize(c.stream));
   finish_statistics(c.s, sums.data(), sums.data() + (size_t)np * 9, profiles, diag);
+  return DUGKS_OK;
+}
+
+int dugks_get_statistics_noalloc(dugks_ctx* ctx, double* profiles, double diag[12], int* rows) {
+  CHECK_CTX();
+  if (!profiles || !diag) return fail(ctx, DUGKS_EARG, "null output");
+  Ctx& c = ctx->c;
+  const int pa = stats_axis(c.s);
+  const int np = c.s.n[pa];
+  if (rows) *rows = np;
+  if (np > kNoAllocStatRows)
+    return fail(ctx, DUGKS_EUNSUPPORTED,
+                "dugks_get_statistics_noalloc: " + std::to_string(np) + " wall-normal rows exceed its limit of " +
+                    std::to_string(kNoAllocStatRows));
+  if (!c.stats_noalloc) return fail(ctx, DUGKS_ESTATE, "no statistics scratch on this device");
+  // the library's per-device scratch (see noalloc_stats_scratch): partial
+  // sums first, the final sums behind them
+  double* out = c.stats_noalloc + stats_scratch_doubles(np) - ((size_t)np * 9 + 12);
+  HIPCHK(launch_stats(c, c.f[c.cur], pa, c.stats_noalloc, out));
+  // the final sums go straight into the caller's buffers, finished in place
+  HIPCHK(hipMemcpyAsync(profiles, out, (size_t)np * 9 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipMemcpyAsync(diag, out + (size_t)np * 9, 12 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  finish_statistics(c.s, profiles, diag, profiles, diag);
   return DUGKS_OK;
 }
 

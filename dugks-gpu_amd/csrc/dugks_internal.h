@@ -74,6 +74,12 @@ bool adjoint_faces_fused_covers(const Ctx& c);
 // u, v, w, uw, uv, vw, uu, vv, ww, then global sums, min and max of
 // rho, u, v, w.  scratch holds stats_scratch_doubles(n_pa) doubles.
 size_t stats_scratch_doubles(int np);
+// Statistics scratch held by the library for dugks_get_statistics_noalloc:
+// one array per GPU for the life of the process, sized for at most
+// kNoAllocStatRows wall-normal rows.  noalloc_stats_scratch resolves its
+// address on `device` (which must be current), once per device.
+constexpr int kNoAllocStatRows = 2048;
+hipError_t noalloc_stats_scratch(int device, double** ptr);
 hipError_t launch_stats(Ctx& c, const void* f, int pa, double* scratch, double* out);
 hipError_t launch_macro(Ctx& c, const void* f, double* rho, double* ux, double* uy, double* uz);
 hipError_t launch_export(Ctx& c, const void* f, double* dst);
@@ -134,6 +140,7 @@ struct Ctx {
   int cur = 0;                // f[cur] holds the current state
   void* fbx = nullptr;        // ghost-inclusive f-bar (two-pass kernels, adjoint)
   double* stats = nullptr;      // planar statistics: partial sums and results
+  double* stats_noalloc = nullptr;  // this device's library-held statistics scratch
   size_t stats_len = 0;
   void* adj_faces = nullptr;    // adjoint face records (19 cotangents, or kCompact values), 3 axes, ghost-inclusive
   int adj_faces_format = -1;    // what adj_faces holds: -1 nothing yet, 0 full records, 1 compact records

@@ -1,5 +1,8 @@
 // Utility kernels: macroscopic output, state export/import, FP64 global
 // sums, constant-flow-rate controller, perturbation forcing, loss functionals.
+#include <map>
+#include <mutex>
+
 #include "device_common.h"
 
 namespace dugks {
@@ -430,6 +433,34 @@ hipError_t loss_t(Ctx& c, const void* f, int kind, double* value, G* grad, doubl
 }  // namespace
 
 size_t stats_scratch_doubles(int np) { return (size_t)np * kStatBlocks * (kStatSums + 8) + (size_t)np * 9 + 12; }
+
+// The scratch of dugks_get_statistics_noalloc: partial sums and final sums
+// of launch_stats for up to kNoAllocStatRows rows (2.9 MB), a file-scope
+// device array, so the entry point allocates nothing.  Each GPU holds its
+// own copy for the life of the process; calls that overlap on one GPU share
+// it and race.  k_stats writes every partial slot k_stats_finish reads, so
+// nothing needs clearing between calls.
+static __device__ __align__(256) double scratch[(size_t)kNoAllocStatRows * kStatBlocks * (kStatSums + 8) +
+                                                (size_t)kNoAllocStatRows * 9 + 12];
+
+hipError_t noalloc_stats_scratch(int device, double** ptr) {
+  // the symbol's host-side name is not a device address: resolve it once
+  // per device (with that device current) and keep the address
+  static std::mutex mu;
+  static std::map<int, double*> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = addr.find(device);
+  if (it != addr.end()) {
+    *ptr = it->second;
+    return hipSuccess;
+  }
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  addr[device] = static_cast<double*>(p);
+  *ptr = static_cast<double*>(p);
+  return hipSuccess;
+}
 
 hipError_t launch_stats(Ctx& c, const void* f, int pa, double* scratch, double* out) {
   if (c.precision == 0) return stats_t<double>(c, f, pa, scratch, out);

@@ -56,6 +56,7 @@ PROTOTYPES = {
     "dugks_get_schedule": (_i, [_ctxp, _ip]),
     "dugks_write_outputs": (_i, [_ctxp, _i, _cp]),
     "dugks_get_statistics": (_i, [_ctxp, _dp, _dp, _ip]),
+    "dugks_get_statistics_noalloc": (_i, [_ctxp, _dp, _dp, _ip]),
     "dugks_velmean": (_i, [_cp, _i, _i, _i, _i, _cp]),
     "dugks_get_spacing": (_i, [_ctxp, _i, _dp]),
     "dugks_get_f": (_i, [_ctxp, _dp]),

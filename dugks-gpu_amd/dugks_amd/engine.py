@@ -208,14 +208,16 @@ class Engine:
     def write_outputs(self, step, dirname="."):
         self._check(self._lib.dugks_write_outputs(self._h, int(step), _b(dirname)))
 
-    def statistics(self):
+    def statistics(self, noalloc=False):
         """Planar FP64 statistics along the wall-normal axis (x without walls),
         computed on the device: (profiles [n][9], diag [12]); columns as in
-        profiles-2.dat and diag.dat."""
+        profiles-2.dat and diag.dat.  noalloc: through
+        dugks_get_statistics_noalloc (library-held scratch, no allocation)."""
         prof = np.empty((max(self.shape), 9))
         diag = np.empty(12)
         rows = ctypes.c_int()
-        self._check(self._lib.dugks_get_statistics(self._h, _ptr(prof), _ptr(diag), ctypes.byref(rows)))
+        fn = self._lib.dugks_get_statistics_noalloc if noalloc else self._lib.dugks_get_statistics
+        self._check(fn(self._h, _ptr(prof), _ptr(diag), ctypes.byref(rows)))
         return prof[:rows.value].copy(), diag
 
     SPECTRAL_KEYS = ("te", "etotal", "eps", "eta", "vk", "tk", "uprime", "taylor", "re_lambda", "L",

@@ -126,6 +126,16 @@ int dugks_write_outputs(dugks_ctx* ctx, int step, const char* dir);
  * diag.dat columns).  Host buffers; synchronises the stream. */
 int dugks_get_statistics(dugks_ctx* ctx, double* profiles, double diag[12], int* rows);
 
+/* dugks_get_statistics without allocating: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results (the same kernels, bit for bit); its scratch is a
+ * library-held device array, one per GPU for the life of the process, so it
+ * calls no hipMalloc or hipFree.  Limit: at most 2048 rows along the
+ * wall-normal axis (DUGKS_EUNSUPPORTED beyond).  Calls that overlap on one
+ * GPU share that scratch and race. */
+int dugks_get_statistics_noalloc(dugks_ctx* ctx, double* profiles, double diag[12], int* rows);
+
 /* Spectral statistics of decaying homogeneous isotropic turbulence, stat_te
  * of the reference's DHIT code (src/fortran/dhit/src/stat.f90:5-205), of the
  * current velocity field (the output velocity of dugks_get_macro), on the
