@@ -91,12 +91,27 @@ PROTOTYPES = {
 _lib = None
 
 
+def _start_torch_runtime():
+    """PyTorch's ROCm wheels carry their own HIP and HSA runtimes beside the
+    system ROCm that libdugks.so links, so a process using both holds two.
+    When the library's starts first, torch's later finds no GPU ("No HIP
+    GPUs are available"); the other order works.  So torch's is started
+    first whenever torch is installed and sees a GPU."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.cuda.init()
+
+
 def load():
     """Load libdugks.so and attach prototypes; raises if it is not built."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"libdugks.so is not built at {LIB_PATH}; run __graft_entry__.build()")
+        _start_torch_runtime()
         lib = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in PROTOTYPES.items():
             fn = getattr(lib, name)
