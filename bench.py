@@ -222,9 +222,12 @@ def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32"):
 def host_cpu():
     """CPU model, the physical cores of the CPUs this process may run on, and
     the host's CPU share for this job (OMP_NUM_THREADS when the launcher sets
-    it): (model, physical_cores, share)."""
+    it): (model, physical_cores, share).  Cores are counted from the sysfs
+    topology (package and core id of each allowed CPU), else from
+    /proc/cpuinfo when it carries those ids, else (or when those name a
+    single core for several CPUs) one per allowed CPU."""
     model = "unknown"
-    core_of = {}
+    info = {}
     cpu, phys, core = None, None, None
     try:
         with open("/proc/cpuinfo") as fh:
@@ -240,17 +243,33 @@ def host_cpu():
                 elif k == "core id":
                     core = v
                 elif not line.strip() and cpu is not None:
-                    core_of[cpu] = (phys, core)
+                    info[cpu] = (phys, core)
                     cpu, phys, core = None, None, None
     except OSError:
         pass
-    allowed = os.sched_getaffinity(0)
-    cores = len({core_of.get(c, ("?", c)) for c in allowed}) or len(allowed)
+    allowed = sorted(os.sched_getaffinity(0))
+
+    def sysfs(c):
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        with open(base + "physical_package_id") as a, open(base + "core_id") as b:
+            return a.read().strip(), b.read().strip()
+
+    try:
+        cores = len({sysfs(c) for c in allowed})
+    except OSError:
+        ids = [info.get(c, (None, None)) for c in allowed]
+        if ids and all(p is not None and k is not None for p, k in ids):
+            cores = len(set(ids))
+        else:
+            cores = len(allowed)
+    if cores == 1 and len(allowed) > 1:
+        # a virtual machine that reports one core id for every CPU
+        cores = len(allowed)
     share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
     return model, cores, share
 
 
-def cpu_baseline(configs, threads, target_s=15.0):
+def cpu_baseline(configs, threads, cpu_host, target_s=15.0):
     """FP64 oracle (the reference algorithm restated in C++/OpenMP) on a
     z-slab of the same workload: 256 x 192 x 32 cells, same mesh and dt,
     stepped in batches of 5 until about target_s seconds of CPU work, one
@@ -277,7 +296,7 @@ def cpu_baseline(configs, threads, target_s=15.0):
         n += 5
     dt = time.perf_counter() - t
     cells = int(np.prod(dims))
-    model, phys, share = host_cpu()
+    model, phys, share = cpu_host
     return dict(value=cells * n / dt / 1e6, unit="MLUPS", cores=threads, kind="port",
                 cpu_model=model, physical_cores=phys, cpu_share=share or None,
                 omp_proc_bind=os.environ.get("OMP_PROC_BIND"),
@@ -460,6 +479,11 @@ def main():
         print("bench.py: this build runs on one GPU only (the domain is not sharded); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    # the host's cores, read before any OpenMP runtime starts: with
+    # OMP_PROC_BIND set, the first parallel region (torch's or the oracle's,
+    # which share libgomp) binds this thread to one core, and the affinity
+    # mask then names that core only
+    cpu_host = host_cpu()
     # the CPU baseline's OpenMP threads stay on their cores; libgomp reads
     # this when it starts, which is after this point
     os.environ.setdefault("OMP_PROC_BIND", "close")
@@ -511,10 +535,10 @@ def main():
     traffic = measured_traffic(primary)
     cpu = None
     if not args.no_cpu:
-        _, phys, share = host_cpu()
+        _, phys, share = cpu_host
         threads = args.cpu_threads or (min(phys, share) if share else phys)
         log(f"cpu baseline on {threads} threads")
-        cpu = cpu_baseline(configs, threads)
+        cpu = cpu_baseline(configs, threads, cpu_host)
         log(f"cpu baseline {cpu['value']:.2f} MLUPS")
     out = {
         "metric": METRIC,
