@@ -135,13 +135,14 @@ def run_mode(dugks_amd, configs, torch, precision, args, np=None):
 
 
 def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32"):
-    """Forward + adjoint over T steps of the same 256x192x256 channel with a
-    constant driving force (the adjoint does not differentiate the flow-rate
-    controller): wall-shear loss, sqrt(T) checkpoints in HBM, gradient with
-    respect to a per-cell body force.  Reports the whole-sweep and per-step
-    adjoint/forward time ratios and the sweep's device memory."""
+    """Forward + adjoint over T steps of the same 256x192x256 channel at
+    constant flow rate (the sweep differentiates the flow-rate controller):
+    wall-shear loss, sqrt(T) checkpoints in HBM, gradient with respect to a
+    per-cell body force.  Reports the per-step adjoint (face pass and gather)
+    and whole-sweep time ratios to the forward, and the sweep's device
+    memory."""
     prec = {"fp64": dugks_amd.PREC_FP64, "fp32": dugks_amd.PREC_FP32}[precision]
-    text = configs.large_channel(precision=precision, flow_rate=0)
+    text = configs.large_channel(precision=precision)
     eng = dugks_amd.Engine(text, precision=prec, device=0, stream=torch.cuda.current_stream().cuda_stream)
     nz, ny, nx = eng.shape[2], eng.shape[1], eng.shape[0]
     f0 = torch.empty((19, nz, ny, nx), dtype=torch.float64, device="cuda")
@@ -193,6 +194,7 @@ def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32"):
     eng.close()
     fwd_step = t_fwd / T
     res = dict(precision=precision, T=T, interval=int(math.ceil(math.sqrt(T))), loss="wall shear",
+               forcing="constant flow rate, controller differentiated",
                forward_ms_per_step=round(fwd_step * 1e3, 4),
                adjoint_lambda=lam_type,
                adjoint_step_ms=round(t_adj[lam_type] * 1e3, 4),
@@ -379,44 +381,51 @@ def study_channel(dugks_amd, configs, torch, np, steps):
     return out
 
 
-def study_adjoint(dugks_amd, configs, torch, np, T, ndirs=32):
+def study_adjoint(dugks_amd, configs, torch, np, T, ndirs=32, flow=False):
     """Adjoint workload: transitional channel 64x64x32 (seeded perturbed
     laminar state, Re = 2000), J = time-mean wall shear stress over T steps,
     control G ~ N(0, 1e-6) per cell (seeded).  sqrt(T)-checkpointed
     gradient in FP64 and FP32, against central differences of J along
     `ndirs` seeded random directions (eps per precision), FP32 against FP64,
-    and the timing of the sweep against T forward steps."""
-    out = {"workload": "transitional channel 64x64x32, Re=2000, wall-shear loss", "T": T, "directions": ndirs}
+    and the timing of the sweep against T forward steps.  flow: driven at
+    constant flow rate instead of constant force (the sweep differentiates
+    the controller; every run restarts it from the same force)."""
+    kw = dict(flow_rate="auto", flow_axis=0) if flow else {}
+    out = {"workload": "transitional channel 64x64x32, Re=2000, wall-shear loss, "
+                       + ("constant flow rate" if flow else "constant force"), "T": T, "directions": ndirs}
     grads, fds = {}, {}
     for m, eps in (("fp64", 1.0), ("fp32", 100.0)):
         prec = {"fp64": dugks_amd.PREC_FP64, "fp32": dugks_amd.PREC_FP32}[m]
-        eng = dugks_amd.Engine(configs.adjoint_channel(precision=m), precision=prec, device=0,
+        eng = dugks_amd.Engine(configs.adjoint_channel(precision=m, **kw), precision=prec, device=0,
                                stream=torch.cuda.current_stream().cuda_stream)
         nx, ny, nz = eng.shape
         f0 = torch.empty((19, nz, ny, nx), dtype=torch.float64, device="cuda")
         eng.get_f_device(f0)
+        F0 = eng.ambient_force()
         G = 1e-6 * np.random.default_rng(2024).standard_normal((3, nz, ny, nx))
         Gd = torch.from_numpy(G).cuda()
         gd = torch.empty_like(Gd)
 
-        def run_J(ctrl):
+        def restart(ctrl):
+            eng.set_ambient_force(F0)
             eng.set_f_device(f0)
             eng.set_cell_force_device(ctrl)
+
+        def run_J(ctrl):
+            restart(ctrl)
             tot = 0.0
             for n in range(T):
                 eng.step(1, istep0=1 + n)
                 tot += eng.loss(dugks_amd.LOSS_WALL_SHEAR)
             return tot / T
 
-        eng.set_f_device(f0)
-        eng.set_cell_force_device(Gd)
+        restart(Gd)
         torch.cuda.synchronize()
         t = time.perf_counter()
         J, peak = eng.forward_adjoint(T, gd)
         torch.cuda.synchronize()
         t_sweep = time.perf_counter() - t
-        eng.set_f_device(f0)
-        eng.set_cell_force_device(Gd)
+        restart(Gd)
         torch.cuda.synchronize()
         t = time.perf_counter()
         eng.step(T, istep0=1)
@@ -466,10 +475,10 @@ def main():
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--adj-steps", type=int, default=1000,
                     help="steps of the forward+adjoint sweep on the large channel (0: skip)")
-    ap.add_argument("--study", choices=["tgv", "channel", "adjoint"],
+    ap.add_argument("--study", choices=["tgv", "channel", "adjoint", "adjoint_flow"],
                     help="instead of the benchmark: the reduced-precision error study on the "
                          "Taylor-Green vortex or the 128^3 channel, or the adjoint gradient study "
-                         "(prints one JSON line)")
+                         "at constant force or constant flow rate (prints one JSON line)")
     ap.add_argument("--study-steps", type=int, default=0, help="steps of the study (default 1000 / 10000)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB read + 1 GiB write elementwise kernel 5 times "
@@ -504,8 +513,9 @@ def main():
         import numpy as np
         if args.study == "tgv":
             res = study_tgv(dugks_amd, configs, torch, np, args.study_steps or 1000)
-        elif args.study == "adjoint":
-            res = study_adjoint(dugks_amd, configs, torch, np, args.study_steps or 400)
+        elif args.study in ("adjoint", "adjoint_flow"):
+            res = study_adjoint(dugks_amd, configs, torch, np, args.study_steps or 400,
+                                flow=args.study == "adjoint_flow")
         else:
             res = study_channel(dugks_amd, configs, torch, np, args.study_steps or 10000)
         print(json.dumps({"study": args.study, **res}), flush=True)

@@ -686,7 +686,8 @@ int gather_z_chunk(const KParams& P) {
 // One adjoint step.  Lam = double: every configuration; Lam = float (FP32
 // arithmetic with the z-marching gather only): lam' and lam in FP32.
 template <class Real, class Store, class Lam>
-hipError_t run_adjoint(Ctx& cx, const void* f, const Lam* lam_next, Lam* lam_prev, double* grad, int wall) {
+hipError_t run_adjoint(Ctx& cx, const void* f, const Lam* lam_next, Lam* lam_prev, double* grad, int wall,
+                       double* fpart, long long fpart_len, long long* nparts) {
   const KParams& P = cx.kp;
   Real* faces = static_cast<Real*>(cx.adj_faces);
   constexpr bool kLam32 = std::is_same<Lam, float>::value;
@@ -706,8 +707,10 @@ hipError_t run_adjoint(Ctx& cx, const void* f, const Lam* lam_next, Lam* lam_pre
     cx.adj_faces_format = fmt;
   }
   const int format = !compact ? kFacesFull : (kLam32 ? kFacesCompactLam32 : kFacesCompact);
-  hipError_t e = launch_adjoint_faces_fused(cx, f, lam_next, faces, format, &handled);
+  hipError_t e = launch_adjoint_faces_fused(cx, f, lam_next, faces, format, &handled, fpart, fpart_len, nparts);
   if (e != hipSuccess) return e;
+  // the force cotangent of the face collisions comes from the fused pass only
+  if (fpart && !handled) return hipErrorNotSupported;
   if constexpr (!std::is_same<Real, double>::value) {
     if (compact) {
       const int TZ = gather_z_chunk(P);
@@ -753,7 +756,8 @@ hipError_t run_adjoint(Ctx& cx, const void* f, const Lam* lam_next, Lam* lam_pre
 namespace {
 
 template <class Lam>
-hipError_t adjoint_step(Ctx& c, const void* f, const Lam* lam_next, Lam* lam_prev, double* grad_force) {
+hipError_t adjoint_step(Ctx& c, const void* f, const Lam* lam_next, Lam* lam_prev, double* grad_force,
+                        double* fpart, long long fpart_len, long long* nparts) {
   const Setup& s = c.s;
   if (s.recon != 0) return hipErrorNotSupported;
   int walls = 0, wall = -1;
@@ -778,29 +782,44 @@ hipError_t adjoint_step(Ctx& c, const void* f, const Lam* lam_next, Lam* lam_pre
   switch (c.precision) {
     case 0:
       if constexpr (std::is_same<Lam, double>::value)
-        return run_adjoint<double, double, Lam>(c, f, lam_next, lam_prev, grad_force, wall);
+        return run_adjoint<double, double, Lam>(c, f, lam_next, lam_prev, grad_force, wall, fpart, fpart_len,
+                                                nparts);
       else
         return hipErrorNotSupported;
     case 1:
-      return run_adjoint<float, float, Lam>(c, f, lam_next, lam_prev, grad_force, wall);
+      return run_adjoint<float, float, Lam>(c, f, lam_next, lam_prev, grad_force, wall, fpart, fpart_len, nparts);
     default:
-      return run_adjoint<float, __half, Lam>(c, f, lam_next, lam_prev, grad_force, wall);
+      return run_adjoint<float, __half, Lam>(c, f, lam_next, lam_prev, grad_force, wall, fpart, fpart_len, nparts);
   }
 }
 
 }  // namespace
 
-hipError_t launch_adjoint(Ctx& c, const void* f, const double* lam_next, double* lam_prev, double* grad_force) {
-  return adjoint_step<double>(c, f, lam_next, lam_prev, grad_force);
+hipError_t launch_adjoint(Ctx& c, const void* f, const double* lam_next, double* lam_prev, double* grad_force,
+                          double* fpart, long long fpart_len, long long* nparts) {
+  return adjoint_step<double>(c, f, lam_next, lam_prev, grad_force, fpart, fpart_len, nparts);
 }
 
-hipError_t launch_adjoint_lam32(Ctx& c, const void* f, const float* lam_next, float* lam_prev, double* grad_force) {
-  return adjoint_step<float>(c, f, lam_next, lam_prev, grad_force);
+hipError_t launch_adjoint_lam32(Ctx& c, const void* f, const float* lam_next, float* lam_prev, double* grad_force,
+                                double* fpart, long long fpart_len, long long* nparts) {
+  return adjoint_step<float>(c, f, lam_next, lam_prev, grad_force, fpart, fpart_len, nparts);
 }
 
 bool adjoint_lam32_supported(const Ctx& c) {
   if (c.precision == 0 || c.precision == 3 || c.s.recon != 0) return false;
   return gather_z_supported(c) && adjoint_faces_fused_covers(c);
+}
+
+// The sweep's adjoint step: FP64 on the fused face pass (full records), or
+// FP32 arithmetic with lam in FP32 (compact records); the forms
+// launch_adjoint_faces_fused gives the force cotangent for.
+bool adjoint_flow_supported(const Ctx& c) {
+  if (c.s.recon != 0) return false;
+  int walls = 0;
+  for (int a = 0; a < 3; a++) walls += !c.s.periodic[a];
+  if (walls > 1) return false;
+  if (c.precision == 0) return adjoint_faces_fused_covers(c);
+  return adjoint_lam32_supported(c);
 }
 
 }  // namespace dugks

@@ -848,9 +848,36 @@ void release_sweep(Ctx& c) {
   hipFree(w.fT);
   hipFree(w.lam[0]);
   hipFree(w.lam[1]);
+  hipFree(w.frec);
+  hipFree(w.ctl);
+  hipFree(w.fpart);
   c.bytes_allocated -= w.bytes;
   w = SweepPool();
 }
+
+// A device buffer of the pool of at least n doubles (reallocated when short).
+hipError_t pool_doubles(Ctx& c, double*& p, size_t& len, size_t n) {
+  if (p && len >= n) return hipSuccess;
+  if (p) {
+    hipFree(p);
+    c.sweep.bytes -= (double)len * sizeof(double);
+    c.bytes_allocated -= (double)len * sizeof(double);
+    p = nullptr;
+    len = 0;
+  }
+  hipError_t e = c.alloc((void**)&p, n * sizeof(double));
+  if (e != hipSuccess) return e;
+  len = n;
+  c.sweep.bytes += (double)n * sizeof(double);
+  return hipSuccess;
+}
+
+// The step kernels leave the flow-rate controller alone while it lives.
+struct FrozenFlow {
+  Ctx& c;
+  explicit FrozenFlow(Ctx& cx) : c(cx) { c.flow_frozen = true; }
+  ~FrozenFlow() { c.flow_frozen = false; }
+};
 
 // Grow a list of state buffers to n entries (existing ones are kept).
 hipError_t pool_states(Ctx& c, std::vector<void*>& v, size_t n, size_t bytes) {
@@ -875,10 +902,14 @@ int dugks_forward_record(dugks_ctx* ctx, int istep0, int T, int loss_kind, int i
   if (T <= 0 || !J || (loss_kind != 0 && loss_kind != 1)) return fail(ctx, DUGKS_EARG, "bad forward_record arguments");
   Ctx& c = ctx->c;
   if (loss_kind == 0 && c.s.wall_axis() < 0) return fail(ctx, DUGKS_EUNSUPPORTED, "wall shear needs a wall axis");
-  if (c.s.flow_rate != 0 || c.s.perturb)
-    return fail(ctx, DUGKS_EUNSUPPORTED, "adjoint with flow-rate control or perturb forcing is not supported");
+  if (c.s.perturb) return fail(ctx, DUGKS_EUNSUPPORTED, "the adjoint does not cover perturb (streak) forcing");
   if (c.precision == DUGKS_PREC_FP16N)
     return fail(ctx, DUGKS_EUNSUPPORTED, "the adjoint does not cover FP16 non-equilibrium storage");
+  const bool flow = c.s.flow_rate != 0;
+  if (flow && !adjoint_flow_supported(c))
+    return fail(ctx, DUGKS_EUNSUPPORTED,
+                "the adjoint of the flow-rate controller needs FP64, or FP32 arithmetic with lam in FP32, on the "
+                "fused face pass and at most one wall axis");
   SweepPool& w = c.sweep;
   w.recorded = false;
   const size_t nf = c.s.ncell() * Q;
@@ -898,6 +929,25 @@ int dugks_forward_record(dugks_ctx* ctx, int istep0, int T, int loss_kind, int i
     if ((rc = chk(c.alloc(&w.fT, sb), "final-state alloc"))) return rc;
     w.bytes += (double)sb;
   }
+  // with the controller: the force before each step (frec[n] drives step n)
+  // and the controller's previous-valid flag when the record begins
+  const int ax = c.s.stream_axis();
+  if (flow) {
+    const int fmt = c.precision == 0 ? kFacesFull : kFacesCompactLam32;
+    if ((rc = chk(pool_doubles(c, w.frec, w.frec_len, (size_t)T + 1), "force record alloc"))) return rc;
+    size_t ctl_len = w.ctl ? (size_t)kFlowCtlLen : 0;
+    if ((rc = chk(pool_doubles(c, w.ctl, ctl_len, kFlowCtlLen), "controller scalars alloc"))) return rc;
+    const long long nb = adjoint_faces_fused_blocks(c, fmt);
+    if ((rc = chk(pool_doubles(c, w.fpart, w.fpart_len, (size_t)std::max(nb, 1LL)), "force partials alloc")))
+      return rc;
+    if ((rc = chk(hipMemcpyAsync(w.ctl + kFlowCtlFlag0, c.famb_dev + 4, sizeof(double), hipMemcpyDeviceToDevice,
+                                 c.stream),
+                  "flag copy")))
+      return rc;
+    if ((rc = chk(hipMemcpyAsync(w.frec, c.famb_dev + ax, sizeof(double), hipMemcpyDeviceToDevice, c.stream),
+                  "force copy")))
+      return rc;
+  }
   // dval[0]: the loss summed over the forward sweep on the device, read once
   double* dval = c.red + reduction_scratch_doubles() + 9;
   if ((rc = chk(launch_fill(c, dval, 0.0, 1), "clear"))) return rc;
@@ -907,6 +957,12 @@ int dugks_forward_record(dugks_ctx* ctx, int istep0, int T, int loss_kind, int i
                     "checkpoint copy")))
         return rc;
     if ((rc = step_once(ctx, istep0 + n))) return rc;
+    // the loss of f^{n+1} reads the force the controller has just set
+    if (flow)
+      if ((rc = chk(hipMemcpyAsync(w.frec + n + 1, c.famb_dev + ax, sizeof(double), hipMemcpyDeviceToDevice,
+                                   c.stream),
+                    "force copy")))
+        return rc;
     if ((rc = chk(launch_loss(c, c.f[c.cur], loss_kind, dval, nullptr, 0, true), "loss"))) return rc;
   }
   if ((rc = chk(hipMemcpyAsync(w.fT, c.f[c.cur], sb, hipMemcpyDeviceToDevice, c.stream), "final copy")))
@@ -921,6 +977,8 @@ int dugks_forward_record(dugks_ctx* ctx, int istep0, int T, int loss_kind, int i
   w.loss_kind = loss_kind;
   w.interval = interval;
   w.ncp = ncp;
+  w.flow = flow;
+  w.replayed = false;
   w.recorded = true;
   return DUGKS_OK;
 }
@@ -969,6 +1027,25 @@ int dugks_adjoint_replay(dugks_ctx* ctx, double* dev_grad_force, double* dev_gra
   }
   if ((rc = chk(launch_fill(c, dev_grad_force, 0.0, (long long)(3 * N)), "clear"))) return rc;
   const double invT = 1.0 / T;
+  // With the flow-rate controller each step runs with its recorded force
+  // and the controller is differentiated through its scalar recurrence
+  // (k_flow_adjoint_scalar): before the adjoint of step n, lam = dJ/df^{n+1}
+  // receives the bulk-velocity cotangent ubar_{n+1} dU/df^{n+1}; the adjoint
+  // step also yields the face part of the force cotangent phi_n, and the
+  // change of the summed cell force gradient gives the rest.
+  const bool flow = w.flow;
+  const int ax = c.s.stream_axis();
+  std::unique_ptr<FrozenFlow> frozen;
+  long long nparts = 0;
+  auto set_force = [&](int n) {
+    return hipMemcpyAsync(c.famb_dev + ax, w.frec + n, sizeof(double), hipMemcpyDeviceToDevice, c.stream);
+  };
+  w.replayed = false;
+  if (flow) {
+    frozen.reset(new FrozenFlow(c));
+    if ((rc = chk(launch_fill(c, w.ctl, 0.0, kFlowCtlFlag0), "clear"))) return rc;
+    if ((rc = chk(launch_fill(c, w.ctl + kFlowCtlLoss, 0.0, kFlowCtlLen - kFlowCtlLoss), "clear"))) return rc;
+  }
   // Recompute each segment from its checkpoint, then step back through it.
   for (int q = w.ncp - 1; q >= 0; q--) {
     const int n0 = q * K, n1 = std::min(T, n0 + K);
@@ -979,16 +1056,37 @@ int dugks_adjoint_replay(dugks_ctx* ctx, double* dev_grad_force, double* dev_gra
     };
     for (int n = n0 + 1; n < n1; n++) {
       bool handled = false;
-      hipError_t e = hipSuccess;
+      hipError_t e = flow ? set_force(n - 1) : hipSuccess;
       void* out = w.seg[n - n0 - 1];
-      if (c.kernel != DUGKS_KERNEL_TWOPASS) e = launch_step_fused(c, state(n - 1), out, &handled);
+      if (e == hipSuccess && c.kernel != DUGKS_KERNEL_TWOPASS) e = launch_step_fused(c, state(n - 1), out, &handled);
       if (e == hipSuccess && !handled) e = launch_step_twopass(c, state(n - 1), out);
       if ((rc = chk(e, "segment recompute"))) return rc;
     }
     for (int n = n1 - 1; n >= n0; n--) {
-      // lam = dJ/df^{n+1}: add the loss term of step n+1, then
-      // lam2 = (df^{n+1}/df^n)^T lam about f^n
-      if (lam32) {
+      // lam = dJ/df^{n+1}: add the loss term of step n+1 (and the
+      // controller's), then lam2 = (df^{n+1}/df^n)^T lam about f^n
+      if (flow) {
+        hipError_t e = hipSuccess;
+        if (n < T - 1) e = launch_axis_sum(c, dev_grad_force, ax, w.ctl + kFlowCtlSumNew);
+        if (e == hipSuccess) e = set_force(n + 1);
+        if (e == hipSuccess)
+          e = lam32 ? launch_loss_lam32(c, state(n + 1), w.loss_kind, w.ctl + kFlowCtlLoss, static_cast<float*>(lam),
+                                        invT, false, ax)
+                    : launch_loss(c, state(n + 1), w.loss_kind, w.ctl + kFlowCtlLoss, static_cast<double*>(lam), invT,
+                                  false, ax);
+        if (e == hipSuccess)
+          e = launch_flow_adjoint_scalar(c, w.ctl, w.fpart, n < T - 1 ? nparts : 0, n == 0, invT);
+        if (e == hipSuccess)
+          e = lam32 ? launch_flow_cotangent_lam32(c, state(n + 1), w.ctl + kFlowCtlUbar, static_cast<float*>(lam))
+                    : launch_flow_cotangent(c, state(n + 1), w.ctl + kFlowCtlUbar, static_cast<double*>(lam));
+        if (e == hipSuccess) e = set_force(n);
+        if (e == hipSuccess)
+          e = lam32 ? launch_adjoint_lam32(c, state(n), static_cast<const float*>(lam), static_cast<float*>(lam2),
+                                           dev_grad_force, w.fpart, (long long)w.fpart_len, &nparts)
+                    : launch_adjoint(c, state(n), static_cast<const double*>(lam), static_cast<double*>(lam2),
+                                     dev_grad_force, w.fpart, (long long)w.fpart_len, &nparts);
+        if ((rc = chk(e, "controller adjoint"))) return rc;
+      } else if (lam32) {
         float *l = static_cast<float*>(lam), *l2 = static_cast<float*>(lam2);
         if ((rc = chk(launch_loss_lam32(c, state(n + 1), w.loss_kind, dval + 1, l, invT), "loss grad"))) return rc;
         if ((rc = chk(launch_adjoint_lam32(c, state(n), l, l2, dev_grad_force), "adjoint"))) return rc;
@@ -1000,12 +1098,20 @@ int dugks_adjoint_replay(dugks_ctx* ctx, double* dev_grad_force, double* dev_gra
       std::swap(lam, lam2);
     }
   }
+  if (flow) {
+    // nu_0 = phi_0 + nu_1 = dJ/dF^0, then the force the forward ended with
+    hipError_t e = launch_axis_sum(c, dev_grad_force, ax, w.ctl + kFlowCtlSumNew);
+    if (e == hipSuccess) e = launch_flow_adjoint_scalar(c, w.ctl, w.fpart, nparts, true, invT);
+    if (e == hipSuccess) e = set_force(T);
+    if ((rc = chk(e, "controller adjoint"))) return rc;
+  }
   if (dev_grad_f0)
     if ((rc = chk(lam32 ? launch_widen(c, static_cast<const float*>(lam), dev_grad_f0, (long long)nf)
                         : hipMemcpyAsync(dev_grad_f0, lam, nf * sizeof(double), hipMemcpyDeviceToDevice, c.stream),
                   "grad copy")))
       return rc;
   if ((rc = chk(hipStreamSynchronize(c.stream), "sync"))) return rc;
+  w.replayed = true;
   if (peak_bytes) {
     // device memory in use at the end of the sweep, when every buffer it
     // needs (pool, adjoint face records, state) is allocated
@@ -1013,6 +1119,18 @@ int dugks_adjoint_replay(dugks_ctx* ctx, double* dev_grad_force, double* dev_gra
     if ((rc = chk(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo"))) return rc;
     peak_bytes[0] = (double)(total_b - free_b);
   }
+  return DUGKS_OK;
+}
+
+int dugks_sweep_force_gradient(dugks_ctx* ctx, double* dJdF0) {
+  CHECK_CTX();
+  if (!dJdF0) return fail(ctx, DUGKS_EARG, "null output");
+  Ctx& c = ctx->c;
+  const SweepPool& w = c.sweep;
+  if (!w.recorded || !w.replayed || !w.flow)
+    return fail(ctx, DUGKS_ESTATE, "no replayed sweep with the flow-rate controller");
+  HIPCHK(hipMemcpyAsync(dJdF0, w.ctl + kFlowCtlNuNext, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
   return DUGKS_OK;
 }
 

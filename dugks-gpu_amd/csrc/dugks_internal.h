@@ -66,10 +66,18 @@ hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled)
 // which the gather uses instead of reading f.
 constexpr int kCompact = 13;
 constexpr int kFacesFull = 0, kFacesCompact = 1, kFacesCompactLam32 = 2;
+// With fpart set, the pass also writes per block the cotangent of the
+// ambient force along the flow axis through the face collisions (nparts
+// values, at most fpart_len, else hipErrorInvalidValue); only FP64 full
+// records and FP32-lam compact records have this form, else
+// hipErrorNotSupported.
 hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, void* faces, int format,
-                                      bool* handled);
+                                      bool* handled, double* fpart = nullptr, long long fpart_len = 0,
+                                      long long* nparts = nullptr);
 // Whether launch_adjoint_faces_fused covers the configuration.
 bool adjoint_faces_fused_covers(const Ctx& c);
+// Number of per-block force partials the pass writes for a format.
+long long adjoint_faces_fused_blocks(const Ctx& c, int format);
 // Planar statistics along axis pa into out (device): [n_pa][9] row sums of
 // u, v, w, uw, uv, vw, uu, vv, ww, then global sums, min and max of
 // rho, u, v, w.  scratch holds stats_scratch_doubles(n_pa) doubles.
@@ -87,21 +95,52 @@ hipError_t launch_import(Ctx& c, const double* src, void* f);
 hipError_t launch_global_sums(Ctx& c, const void* f, double* dev_out8);
 hipError_t launch_flow_control(Ctx& c, const void* f);
 hipError_t launch_perturb_force(Ctx& c, int step);
+// fpart: as launch_adjoint_faces_fused (the face part of the ambient-force
+// cotangent); hipErrorNotSupported where the fused face pass does not run.
 hipError_t launch_adjoint(Ctx& c, const void* f, const double* lam_next, double* lam_prev,
-                          double* grad_force);
+                          double* grad_force, double* fpart = nullptr, long long fpart_len = 0,
+                          long long* nparts = nullptr);
 // The same with lam' and lam in FP32; only FP32 arithmetic on the z-marching
 // gather's configurations (adjoint_lam32_supported), else hipErrorNotSupported.
 hipError_t launch_adjoint_lam32(Ctx& c, const void* f, const float* lam_next, float* lam_prev,
-                                double* grad_force);
+                                double* grad_force, double* fpart = nullptr, long long fpart_len = 0,
+                                long long* nparts = nullptr);
 bool adjoint_lam32_supported(const Ctx& c);
+// Whether the adjoint covers the flow-rate controller (the ambient-force
+// cotangent of the face collisions is available).
+bool adjoint_flow_supported(const Ctx& c);
 // Loss value into dev_value (added to it when value_acc) and, when dev_grad
 // is set, its gradient dJ/df written into dev_grad (grad_acc == 0) or added
 // to it times grad_acc.
+// force_axis >= 0: dev_value[1] also receives dJ/dF of the ambient force
+// component force_axis (the loss reads u = j/rho + h F), under the same
+// value_acc rule.
 hipError_t launch_loss(Ctx& c, const void* f, int kind, double* dev_value, double* dev_grad, double grad_acc = 0,
-                       bool value_acc = false);
+                       bool value_acc = false, int force_axis = -1);
 // The same with the gradient in FP32 (FP32 arithmetic only).
 hipError_t launch_loss_lam32(Ctx& c, const void* f, int kind, double* dev_value, float* dev_grad,
-                             double grad_acc = 0, bool value_acc = false);
+                             double grad_acc = 0, bool value_acc = false, int force_axis = -1);
+// Adjoint of the flow-rate controller.  The controller reads the bulk
+// velocity U = sum (j_a/rho) dV / V of the new state; its cotangent ubar
+// (device scalar) adds ubar (dV/V) (c_ia - u_a)/rho to lam of every
+// population of every cell.
+hipError_t launch_flow_cotangent(Ctx& c, const void* f, const double* ubar, double* lam);
+hipError_t launch_flow_cotangent_lam32(Ctx& c, const void* f, const double* ubar, float* lam);
+// out[0] = sum over cells of v[axis N + cell] (fixed order).
+hipError_t launch_axis_sum(Ctx& c, const double* v, int axis, double* out);
+// Device scalars of the controller's reverse recurrence (SweepPool::ctl):
+// nu = dJ/dF of the force after the current one (dJ/dF^0 when the replay
+// ends), the summed cell force gradient after the last adjoint step and its
+// fresh value, the bulk-velocity cotangent to apply, the controller's
+// previous-valid flag when the record began, and the loss value (discarded)
+// followed by the loss's dJ/dF.
+constexpr int kFlowCtlNuNext = 0, kFlowCtlSum = 1, kFlowCtlSumNew = 2, kFlowCtlUbar = 3, kFlowCtlFlag0 = 4,
+              kFlowCtlLoss = 5, kFlowCtlLen = 8;
+// One reverse step of the controller's scalar recurrence over ctl; step0:
+// the bulk velocity being differentiated is that of step 1 (the flag then
+// comes from the recorded state).
+hipError_t launch_flow_adjoint_scalar(Ctx& c, double* ctl, const double* fpart, long long nparts, bool step0,
+                                      double invT);
 // dst[i] = src[i] (FP32 to FP64), n values.
 hipError_t launch_widen(Ctx& c, const float* src, double* dst, long long n);
 hipError_t launch_axpy(Ctx& c, double a, const double* x, double* y, long long n);
@@ -124,6 +163,17 @@ struct SweepPool {
   // the recorded forward sweep
   bool recorded = false;
   int istep0 = 0, T = 0, loss_kind = 0, interval = 0, ncp = 0;
+  // with the flow-rate controller: the ambient force along the flow axis
+  // before each step (frec[n] drove step n, frec[T] is the final one), the
+  // controller's scalars and the per-block face partials of the force
+  // cotangent
+  bool flow = false;
+  bool replayed = false;  // the reverse sweep of the record has run (ctl holds dJ/dF^0)
+  double* frec = nullptr;
+  size_t frec_len = 0;
+  double* ctl = nullptr;
+  double* fpart = nullptr;
+  size_t fpart_len = 0;
 };
 
 // Library context.
@@ -162,6 +212,9 @@ struct Ctx {
   int fused_tz = 0;
   double* flow_partial = nullptr;
   size_t flow_partial_len = 0;
+  // the step kernels leave the flow-rate controller alone (the adjoint's
+  // replay, which sets the recorded force itself)
+  bool flow_frozen = false;
   SweepPool sweep;
 
   hipError_t alloc(void** p, size_t bytes) {

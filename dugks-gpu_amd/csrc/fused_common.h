@@ -313,8 +313,12 @@ struct Face {
   // The transposed collision up to its equilibrium coefficients: the
   // cotangent of reconstructed value p is fw1 mu_p + Ac + B.c_p (mu_p = 0
   // for the populations that do not cross), before the bounce transpose.
+  // FORCE: also adds the cotangent of the face's force F to gF[0..2]
+  // (through the equilibrium's explicit F and through u = j/rho + (h/2) F).
+  template <bool FORCE = false>
   __device__ __forceinline__ void adjoint_coeffs(const Real F[3], Real hdt, Real fw0, Real fw2,
-                                                 const Real (&mu)[10], Real& Ac, Real (&B)[3]) {
+                                                 const Real (&mu)[10], Real& Ac, Real (&B)[3],
+                                                 double* gF = nullptr) {
     moments();
     const Real inv = rcp_real(rho);
     const Real hh = Real(0.5) * hdt;
@@ -332,6 +336,11 @@ struct Face {
     B[0] = g_u[0] * inv;
     B[1] = g_u[1] * inv;
     B[2] = g_u[2] * inv;
+    if constexpr (FORCE) {
+      gF[0] += (double)(g_F[0] + hh * g_u[0]);
+      gF[1] += (double)(g_F[1] + hh * g_u[1]);
+      gF[2] += (double)(g_F[2] + hh * g_u[2]);
+    }
   }
   // Transpose of collide (after bounce) and bounce: given the cotangent mu
   // of the ten outputs, the cotangent of the 19 reconstructed values before
@@ -339,10 +348,12 @@ struct Face {
   // of the equilibrium term, so lam_p = fw1 mu_p + A + B.c_p; a population
   // that entered through a wall took its opposite's value, so its cotangent
   // moves to that opposite.
+  template <bool FORCE = false>
   __device__ __forceinline__ void adjoint(const Real F[3], Real hdt, Real fw0, Real fw1, Real fw2,
-                                          const Real (&mu)[10], bool low_wall, bool high_wall, Real (&lam)[Q]) {
+                                          const Real (&mu)[10], bool low_wall, bool high_wall, Real (&lam)[Q],
+                                          double* gF = nullptr) {
     Real Ac, B[3];
-    adjoint_coeffs(F, hdt, fw0, fw2, mu, Ac, B);
+    adjoint_coeffs<FORCE>(F, hdt, fw0, fw2, mu, Ac, B, gF);
     expand_cotangents<Real, A>(mu, Ac, B, fw1, low_wall, high_wall, lam);
   }
 };

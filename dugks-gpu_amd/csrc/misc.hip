@@ -188,12 +188,15 @@ __global__ void __launch_bounds__(256) k_perturb(KParams P, int step, double xwi
 // The wall-shear loss only involves the two wall planes: unless the gradient
 // array is being initialised (grad set, gacc == 0), the kernel visits those
 // planes only (2 nx nz of the N cells on a y-wall channel).
-template <class Store, class G>
+// NV = 2: the second sum is dJ/dF_fa, the derivative with respect to the
+// ambient force component fa (both losses read h F).
+template <class Store, class G, int NV = 1>
 __global__ void __launch_bounds__(kRedThreads) k_loss(KParams P, const Store* __restrict__ f, int kind,
                                                       int aw, int as, double visc, double vol,
                                                       double nplane, G* grad, double gacc,
-                                                      double* partial) {
-  double acc[1] = {0};
+                                                      double* partial, int fa = 0) {
+  double acc[NV];
+  for (int q = 0; q < NV; q++) acc[q] = 0;
   double F[3];
   ambient_d(P, F);
   const bool wall_only = kind == 0 && !(grad && gacc == 0.0);
@@ -240,6 +243,14 @@ __global__ void __launch_bounds__(kRedThreads) k_loss(KParams P, const Store* __
         }
       });
       acc[0] += val;
+      if constexpr (NV == 2) {
+        // d/dF_as of coef sum_i (c_is + h F_as) f_i is coef h rho
+        if (fa == as) {
+          double r = x[0];
+          for (int ip = 1; ip < Q; ip++) r += x[ip];
+          acc[1] += coef * P.hdt * r;
+        }
+      }
     } else {
       const double dV = P.wd[0][idx[0] + 1] * P.wd[1][idx[1] + 1] * P.wd[2][idx[2] + 1];
       double r, j[3];
@@ -250,6 +261,7 @@ __global__ void __launch_bounds__(kRedThreads) k_loss(KParams P, const Store* __
         uu += u[a] * u[a];
       }
       acc[0] += 0.5 * r * uu * dV / vol;
+      if constexpr (NV == 2) acc[1] += r * u[fa] * P.hdt * dV / vol;
       if (grad) {
         static_for<Q>([&](auto I) {
           constexpr int ip = decltype(I)::value;
@@ -264,7 +276,7 @@ __global__ void __launch_bounds__(kRedThreads) k_loss(KParams P, const Store* __
       }
     }
   }
-  block_reduce_store<1>(acc, partial);
+  block_reduce_store<NV>(acc, partial);
 }
 
 __global__ void __launch_bounds__(256) k_axpy(double a, const double* __restrict__ x, double* y, long long n) {
@@ -414,20 +426,92 @@ hipError_t sums_t(Ctx& c, const void* f, double* out) {
 }
 
 template <class Store, class G>
-hipError_t loss_t(Ctx& c, const void* f, int kind, double* value, G* grad, double gacc, bool vacc) {
+hipError_t loss_t(Ctx& c, const void* f, int kind, double* value, G* grad, double gacc, bool vacc, int fa) {
   const Setup& s = c.s;
   const int aw = s.wall_axis();
   int as = s.stream_axis();
   if (kind == 0 && aw < 0) return hipErrorInvalidValue;
   if (kind == 0 && as == aw) as = (aw + 1) % 3;
+  if (fa > 2) return hipErrorInvalidValue;
   double vol = s.width[0] * s.width[1] * s.width[2];
   double nplane = aw >= 0 ? (double)s.ncell() / s.n[aw] : 1.0;
-  hipLaunchKernelGGL((k_loss<Store, G>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.stream, c.kp,
-                     static_cast<const Store*>(f), kind, aw < 0 ? 0 : aw, as, s.visc, vol, nplane, grad,
-                     gacc, c.red);
-  hipLaunchKernelGGL((k_reduce_final<1>), dim3(1), dim3(kRedThreads), 0, c.stream, c.red, kRedBlocks,
-                     value, vacc ? 1 : 0);
+  if (fa < 0) {
+    hipLaunchKernelGGL((k_loss<Store, G>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.stream, c.kp,
+                       static_cast<const Store*>(f), kind, aw < 0 ? 0 : aw, as, s.visc, vol, nplane, grad,
+                       gacc, c.red, 0);
+    hipLaunchKernelGGL((k_reduce_final<1>), dim3(1), dim3(kRedThreads), 0, c.stream, c.red, kRedBlocks,
+                       value, vacc ? 1 : 0);
+  } else {
+    hipLaunchKernelGGL((k_loss<Store, G, 2>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.stream, c.kp,
+                       static_cast<const Store*>(f), kind, aw < 0 ? 0 : aw, as, s.visc, vol, nplane, grad,
+                       gacc, c.red, fa);
+    hipLaunchKernelGGL((k_reduce_final<2>), dim3(1), dim3(kRedThreads), 0, c.stream, c.red, kRedBlocks,
+                       value, vacc ? 1 : 0);
+  }
   return hipGetLastError();
+}
+
+// Adjoint of the bulk velocity U = sum (j_a/rho) dV / V that flow_update
+// reads from the new state: lam_i += ubar dU/df_i = ubar (dV/V) (c_ia - u_a)/rho,
+// u_a = j_a/rho, in every cell.
+template <class Store, class G>
+__global__ void __launch_bounds__(256) k_flow_cotangent(KParams P, const Store* __restrict__ f, int axis,
+                                                        double vol, const double* __restrict__ ubar,
+                                                        G* __restrict__ lam) {
+  const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (c >= P.N) return;
+  int i, j, k;
+  cell_coords(P, c, i, j, k);
+  const double dV = P.wd[0][i + 1] * P.wd[1][j + 1] * P.wd[2][k + 1];
+  double x[Q], r, jm[3];
+  load_cell<double, Store>(f, c, P, x);
+  cell_moments<double>(x, r, jm);
+  const double ua = jm[axis] / r;
+  const double coef = ubar[0] * dV / (vol * r);
+  static_for<Q>([&](auto I) {
+    constexpr int ip = decltype(I)::value;
+    const double ca = axis == 0 ? CX[ip] : (axis == 1 ? CY[ip] : CZ[ip]);
+    lam[ip * P.N + c] = (G)((double)lam[ip * P.N + c] + coef * (ca - ua));
+  });
+}
+
+// Per-block sums of v[axis N + cell] over the cells (fixed order).
+__global__ void __launch_bounds__(kRedThreads) k_axis_sum(long long N, const double* __restrict__ v, int axis,
+                                                          double* partial) {
+  double acc[1] = {0};
+  const double* a = v + (long long)axis * N;
+  for (long long c = (long long)blockIdx.x * kRedThreads + threadIdx.x; c < N;
+       c += (long long)kRedBlocks * kRedThreads)
+    acc[0] += a[c];
+  block_reduce_store<1>(acc, partial);
+}
+
+// One reverse step of the controller's scalar recurrence.  After step n,
+// flow_update sets F^{n+1} = F^n - flag_n (U_{n+1} - U_n)/dt
+// + g (target - U_{n+1})/dt (flag_n: a previous U exists; 1 after the first
+// controlled step).  With nu_m = dJ/dF^m and phi_m the cotangent of F^m
+// through step m (the face partials plus the change of the summed cell force
+// gradient):
+//   nu_{n+1} = phi_{n+1} + nu_{n+2} + dJ/dF^{n+1} of the loss at n+1,
+//   ubar_{n+1} = -(flag_n + g)/dt nu_{n+1} + nu_{n+2}/dt.
+// One block.
+__global__ void __launch_bounds__(kRedThreads) k_flow_adjoint_scalar(double* ctl, const double* __restrict__ fpart,
+                                                                     long long nparts, int step0, double gain,
+                                                                     double dt, double invT) {
+  double s[1] = {0};
+  for (long long b = threadIdx.x; b < nparts; b += kRedThreads) s[0] += fpart[b];
+  constexpr AllSums<1> sums;
+  block_reduce<1, kRedThreads>(s, sums.op);
+  if (threadIdx.x == 0) {
+    const double phi = s[0] + (ctl[kFlowCtlSumNew] - ctl[kFlowCtlSum]);
+    const double nu_next = ctl[kFlowCtlNuNext];
+    const double nu = phi + nu_next + invT * ctl[kFlowCtlLoss + 1];
+    const double flag = step0 ? ctl[kFlowCtlFlag0] : 1.0;
+    ctl[kFlowCtlUbar] = -(flag + gain) / dt * nu + nu_next / dt;
+    ctl[kFlowCtlNuNext] = nu;
+    ctl[kFlowCtlSum] = ctl[kFlowCtlSumNew];
+    ctl[kFlowCtlLoss + 1] = 0;
+  }
 }
 
 }  // namespace
@@ -515,18 +599,20 @@ hipError_t launch_perturb_force(Ctx& c, int step) {
 }
 
 hipError_t launch_loss(Ctx& c, const void* f, int kind, double* dev_value, double* dev_grad, double grad_acc,
-                       bool value_acc) {
-  if (c.precision == 0) return loss_t<double, double>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
-  if (c.precision == 1) return loss_t<float, double>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
-  if (c.precision == 2) return loss_t<__half, double>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
-  return loss_t<NeqHalf, double>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
+                       bool value_acc, int force_axis) {
+  const int fa = force_axis;
+  if (c.precision == 0) return loss_t<double, double>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc, fa);
+  if (c.precision == 1) return loss_t<float, double>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc, fa);
+  if (c.precision == 2) return loss_t<__half, double>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc, fa);
+  return loss_t<NeqHalf, double>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc, fa);
 }
 
 hipError_t launch_loss_lam32(Ctx& c, const void* f, int kind, double* dev_value, float* dev_grad, double grad_acc,
-                             bool value_acc) {
+                             bool value_acc, int force_axis) {
+  const int fa = force_axis;
   if (c.precision == 0) return hipErrorNotSupported;
-  if (c.precision == 1) return loss_t<float, float>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
-  if (c.precision == 2) return loss_t<__half, float>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc);
+  if (c.precision == 1) return loss_t<float, float>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc, fa);
+  if (c.precision == 2) return loss_t<__half, float>(c, f, kind, dev_value, dev_grad, grad_acc, value_acc, fa);
   return hipErrorNotSupported;
 }
 
@@ -543,6 +629,52 @@ hipError_t launch_fill(Ctx& c, double* dst, double v, long long n) {
 
 hipError_t launch_widen(Ctx& c, const float* src, double* dst, long long n) {
   hipLaunchKernelGGL(k_widen, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, src, dst, n);
+  return hipGetLastError();
+}
+
+template <class G>
+hipError_t flow_cotangent_t(Ctx& c, const void* f, const double* ubar, G* lam) {
+  const Setup& s = c.s;
+  const double vol = s.width[0] * s.width[1] * s.width[2];
+  const unsigned nb = (unsigned)((c.kp.N + 255) / 256);
+  const int ax = s.stream_axis();
+  switch (c.precision) {
+    case 0:
+      hipLaunchKernelGGL((k_flow_cotangent<double, G>), dim3(nb), dim3(256), 0, c.stream, c.kp,
+                         static_cast<const double*>(f), ax, vol, ubar, lam);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_flow_cotangent<float, G>), dim3(nb), dim3(256), 0, c.stream, c.kp,
+                         static_cast<const float*>(f), ax, vol, ubar, lam);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_flow_cotangent<__half, G>), dim3(nb), dim3(256), 0, c.stream, c.kp,
+                         static_cast<const __half*>(f), ax, vol, ubar, lam);
+      break;
+    default:
+      return hipErrorNotSupported;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_flow_cotangent(Ctx& c, const void* f, const double* ubar, double* lam) {
+  return flow_cotangent_t<double>(c, f, ubar, lam);
+}
+
+hipError_t launch_flow_cotangent_lam32(Ctx& c, const void* f, const double* ubar, float* lam) {
+  return flow_cotangent_t<float>(c, f, ubar, lam);
+}
+
+hipError_t launch_axis_sum(Ctx& c, const double* v, int axis, double* out) {
+  hipLaunchKernelGGL(k_axis_sum, dim3(kRedBlocks), dim3(kRedThreads), 0, c.stream, c.kp.N, v, axis, c.red);
+  hipLaunchKernelGGL((k_reduce_final<1>), dim3(1), dim3(kRedThreads), 0, c.stream, c.red, kRedBlocks, out, 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_flow_adjoint_scalar(Ctx& c, double* ctl, const double* fpart, long long nparts, bool step0,
+                                      double invT) {
+  hipLaunchKernelGGL(k_flow_adjoint_scalar, dim3(1), dim3(kRedThreads), 0, c.stream, ctl, fpart, nparts,
+                     step0 ? 1 : 0, c.s.flow_gain, c.s.dt, invT);
   return hipGetLastError();
 }
 

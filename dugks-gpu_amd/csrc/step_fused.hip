@@ -51,8 +51,11 @@ struct AdjFaceIO {
 // ADJ = true: the adjoint-face pass of the step taken from fin: every face is
 // reconstructed and its collision linearised as in the step, and the
 // cotangents of its 19 reconstructed values, driven by lam' (adj.lam), are
-// written to adj.faces; nothing else is written.
-template <class Real, class Store, int TX, int TY, bool ADJ = false, bool COMPACT = false, class LamT = double>
+// written to adj.faces; nothing else is written.  FADJ (with ADJ): also the
+// cotangent of the ambient force along the flow axis through the face
+// collisions, summed per block into partial[tile] (fixed order).
+template <class Real, class Store, int TX, int TY, bool ADJ = false, bool COMPACT = false, class LamT = double,
+          bool FADJ = false>
 // FP32 arithmetic: two blocks per CU (at most 256 registers per lane)
 __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64, sizeof(Real) == 4 ? 2 : 1) k_fused(
     KParams P, const Store* __restrict__ fin,
@@ -168,6 +171,8 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
   Real fbm[5], fbc[Q], fbn[Q], zm[10];
   double flow = 0;
   const int as = P.flow_axis;
+  // FADJ: the face collisions' force cotangents of this thread's faces
+  double fsum[3] = {0, 0, 0};
 
   // Zn(p+1/2) of this column into An: z-normal lerp between fb(p) and fb(p+1)
   auto store_zn = [&](const Real (&lo)[Q], const Real (&hi)[Q]) {
@@ -438,11 +443,11 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
         const long long fs = ((long long)(p + 1) * e1 + (gj + 1)) * e0 + slot;
         if constexpr (COMPACT) {
           Real Ac, B[3], rec[kCompact];
-          fc.adjoint_coeffs(Famb, hdt, fw0, fw2, mu, Ac, B);
+          fc.template adjoint_coeffs<FADJ>(Famb, hdt, fw0, fw2, mu, Ac, B, fsum);
           compact_record<Real, 0>(mu, Ac, B, fw1, rec);
           store_compact(0, fs, rec);
         } else {
-          fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, lw, hw, lam);
+          fc.template adjoint<FADJ>(Famb, hdt, fw0, fw1, fw2, mu, lw, hw, lam, fsum);
           store_face(0, fs, lam);
         }
       } else {
@@ -490,11 +495,11 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
         const long long fs = ((long long)(p + 1) * e1 + slot) * e0 + (gi + 1);
         if constexpr (COMPACT) {
           Real Ac, B[3], rec[kCompact];
-          fc.adjoint_coeffs(Famb, hdt, fw0, fw2, mu, Ac, B);
+          fc.template adjoint_coeffs<FADJ>(Famb, hdt, fw0, fw2, mu, Ac, B, fsum);
           compact_record<Real, 1>(mu, Ac, B, fw1, rec);
           store_compact(1, fs, rec);
         } else {
-          fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, lw, hw, lam);
+          fc.template adjoint<FADJ>(Famb, hdt, fw0, fw1, fw2, mu, lw, hw, lam, fsum);
           store_face(1, fs, lam);
         }
       } else {
@@ -524,11 +529,11 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
         const long long fs = ((long long)(p + 1) * e1 + (gj + 1)) * e0 + (gi + 1);
         if constexpr (COMPACT) {
           Real Ac, B[3], rec[kCompact];
-          fc.adjoint_coeffs(Famb, hdt, fw0, fw2, mu, Ac, B);
+          fc.template adjoint_coeffs<FADJ>(Famb, hdt, fw0, fw2, mu, Ac, B, fsum);
           compact_record<Real, 2>(mu, Ac, B, fw1, rec);
           store_compact(2, fs, rec);
         } else {
-          fc.adjoint(Famb, hdt, fw0, fw1, fw2, mu, false, false, lam);
+          fc.template adjoint<FADJ>(Famb, hdt, fw0, fw1, fw2, mu, false, false, lam, fsum);
           store_face(2, fs, lam);
         }
       }
@@ -622,6 +627,16 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
     block_reduce<1, T::NT>(v, kSum);
     if (t == 0) partial[tt] = v[0];
   }
+  if constexpr (ADJ && FADJ) {
+    // the flow-axis component (assigned, not selected as an lvalue: a select
+    // between array elements can keep the array in scratch memory)
+    double v[1] = {fsum[0]};
+    if (as == 1) v[0] = fsum[1];
+    if (as == 2) v[0] = fsum[2];
+    constexpr int kSum[1] = {0};
+    block_reduce<1, T::NT>(v, kSum);
+    if (t == 0) partial[tt] = v[0];
+  }
 }
 
 // Final reduction of the per-block flow sums <j/rho> dV and the flow-rate
@@ -662,7 +677,7 @@ hipError_t run_fused(Ctx& cx, const void* fin, void* fout) {
   dim3 grid(P.n[0] / TX, P.n[1] / TY, (P.n[2] + TZ - 1) / TZ);
   const long long nblocks = (long long)grid.x * grid.y * grid.z;
   double* partial = nullptr;
-  const bool flow = cx.s.flow_rate != 0;
+  const bool flow = cx.s.flow_rate != 0 && !cx.flow_frozen;
   if (flow) {
     if ((size_t)nblocks > cx.flow_partial_len) return hipErrorInvalidValue;
     partial = cx.flow_partial;
@@ -681,14 +696,28 @@ hipError_t run_fused(Ctx& cx, const void* fin, void* fout) {
   return hipGetLastError();
 }
 
-template <class Real, class Store, int TX, int TY, bool COMPACT = false, class LamT = double>
-hipError_t run_adj_faces(Ctx& cx, const void* f, const void* lam, void* faces) {
+template <int TX, int TY>
+dim3 adj_faces_grid(const Ctx& cx, int* TZ) {
+  const KParams& P = cx.kp;
+  *TZ = z_chunk(cx, TX, TY);
+  return dim3(P.n[0] / TX, P.n[1] / TY, (P.n[2] + *TZ - 1) / *TZ);
+}
+
+// FADJ: also the per-block force partials into fpart (nparts of them).
+template <class Real, class Store, int TX, int TY, bool COMPACT = false, class LamT = double, bool FADJ = false>
+hipError_t run_adj_faces(Ctx& cx, const void* f, const void* lam, void* faces, double* fpart = nullptr,
+                         long long fpart_len = 0, long long* nparts = nullptr) {
   const KParams& P = cx.kp;
   using T = Tile<TX, TY>;
-  const int TZ = z_chunk(cx, TX, TY);
-  dim3 grid(P.n[0] / TX, P.n[1] / TY, (P.n[2] + TZ - 1) / TZ);
-  hipLaunchKernelGGL((k_fused<Real, Store, TX, TY, true, COMPACT, LamT>), grid, dim3(T::NT), 0, cx.stream, P,
-                     static_cast<const Store*>(f), static_cast<Store*>(nullptr), TZ, nullptr,
+  int TZ = 0;
+  const dim3 grid = adj_faces_grid<TX, TY>(cx, &TZ);
+  if constexpr (FADJ) {
+    const long long nb = (long long)grid.x * grid.y * grid.z;
+    if (nb > fpart_len) return hipErrorInvalidValue;
+    *nparts = nb;
+  }
+  hipLaunchKernelGGL((k_fused<Real, Store, TX, TY, true, COMPACT, LamT, FADJ>), grid, dim3(T::NT), 0, cx.stream, P,
+                     static_cast<const Store*>(f), static_cast<Store*>(nullptr), TZ, FADJ ? fpart : nullptr,
                      AdjFaceIO{lam, faces});
   return hipGetLastError();
 }
@@ -747,7 +776,7 @@ bool adjoint_faces_fused_covers(const Ctx& c) {
 }
 
 hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, void* faces, int format,
-                                      bool* handled) {
+                                      bool* handled, double* fpart, long long fpart_len, long long* nparts) {
   int tx = 32, ty = 4;
   *handled = false;
   if (!adjoint_faces_fused_covers(c) || (format != kFacesFull && c.precision == 0)) return hipSuccess;
@@ -758,6 +787,20 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, vo
   // (16-wide rows write half lines; 1.67 against 1.87 ms on the 256x192x256
   // channel despite the larger halo)
   const bool tall = tx == 16 && format == kFacesFull;
+  if (fpart) {
+    // with the ambient-force cotangent: the formats of the forward-adjoint
+    // sweep (FP64, and FP32 arithmetic with lam in FP32)
+    if (c.precision == 0 && format == kFacesFull)
+      return tall ? run_adj_faces<double, double, 16, 12, false, double, true>(c, f, lam, faces, fpart, fpart_len,
+                                                                                nparts)
+                  : run_adj_faces<double, double, 32, 4, false, double, true>(c, f, lam, faces, fpart, fpart_len,
+                                                                               nparts);
+    if (c.precision == 1 && format == kFacesCompactLam32 && !tall)
+      return run_adj_faces<float, float, 32, 4, true, float, true>(c, f, lam, faces, fpart, fpart_len, nparts);
+    if (c.precision == 2 && format == kFacesCompactLam32 && !tall)
+      return run_adj_faces<float, __half, 32, 4, true, float, true>(c, f, lam, faces, fpart, fpart_len, nparts);
+    return hipErrorNotSupported;
+  }
   switch (c.precision) {
     case 0:
       return tall ? run_adj_faces<double, double, 16, 12>(c, f, lam, faces)
@@ -781,6 +824,13 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, vo
       return tall ? run_adj_faces<float, __half, 16, 12>(c, f, lam, faces)
                   : run_adj_faces<float, __half, 32, 4>(c, f, lam, faces);
   }
+}
+
+long long adjoint_faces_fused_blocks(const Ctx& c, int format) {
+  int tx = 32, ty = 4, TZ = 0;
+  if (!fused_supported(c, &tx, &ty)) return 0;
+  const dim3 g = tx == 16 && format == kFacesFull ? adj_faces_grid<16, 12>(c, &TZ) : adj_faces_grid<32, 4>(c, &TZ);
+  return (long long)g.x * g.y * g.z;
 }
 
 }  // namespace dugks

@@ -77,6 +77,7 @@ PROTOTYPES = {
     "dugks_forward_adjoint": (_i, [_ctxp, _i, _i, _i, _i, _dp, _vp, _vp, _dp]),
     "dugks_forward_record": (_i, [_ctxp, _i, _i, _i, _i, _dp]),
     "dugks_adjoint_replay": (_i, [_ctxp, _vp, _vp, _dp]),
+    "dugks_sweep_force_gradient": (_i, [_ctxp, _dp]),
     "dugks_release_sweep": (_i, [_ctxp]),
     "dugks_sweep_bytes": (_i, [_ctxp, _dp]),
     # host-only helpers (no device needed)

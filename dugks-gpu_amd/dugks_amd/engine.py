@@ -269,6 +269,14 @@ class Engine:
                                                    ctypes.byref(peak)))
         return peak.value
 
+    def sweep_force_gradient(self):
+        """After adjoint_replay of a sweep recorded with the flow-rate
+        controller: dJ/dF^0, the derivative with respect to the ambient force
+        along the flow axis when the record began."""
+        v = ctypes.c_double()
+        self._check(self._lib.dugks_sweep_force_gradient(self._h, ctypes.byref(v)))
+        return v.value
+
     def release_sweep(self):
         self._check(self._lib.dugks_release_sweep(self._h))
 

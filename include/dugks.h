@@ -250,12 +250,23 @@ int dugks_loss(dugks_ctx* ctx, int kind, double* value, double* dev_grad);
  * of its buffers are allocated.  The pool stays allocated for the next
  * sweep until dugks_release_sweep or dugks_free.
  *
+ * With the flow-rate controller on (flow_rate != 0) the forward also records
+ * the ambient force before each step, and the reverse sweep differentiates
+ * the controller: each step's force is a function of the bulk velocities
+ * the controller read, so the gradients include the controller's response.
+ * This needs FP64, or FP32 arithmetic whose adjoint keeps lam in FP32, on
+ * the fused face pass (DUGKS_EUNSUPPORTED otherwise); streak forcing
+ * (perturb) is not covered.  dugks_sweep_force_gradient then gives dJ/dF^0,
+ * the derivative with respect to the ambient force along the flow axis when
+ * the record began.
+ *
  * dugks_forward_adjoint is the two in one call. */
 int dugks_forward_record(dugks_ctx* ctx, int istep0, int T, int loss_kind, int interval, double* J);
 int dugks_adjoint_replay(dugks_ctx* ctx, double* dev_grad_force, double* dev_grad_f0, double* peak_bytes);
 int dugks_forward_adjoint(dugks_ctx* ctx, int istep0, int T, int loss_kind, int interval,
                           double* J, double* dev_grad_force, double* dev_grad_f0,
                           double* peak_bytes);
+int dugks_sweep_force_gradient(dugks_ctx* ctx, double* dJdF0);
 /* Frees the checkpoint pool; *bytes of dugks_sweep_bytes is its current size. */
 int dugks_release_sweep(dugks_ctx* ctx);
 int dugks_sweep_bytes(dugks_ctx* ctx, double* bytes);

@@ -285,6 +285,113 @@ def test_forward_adjoint_matches_chained_oracle_adjoint(rng):
     assert np.abs(_aos(g0.cpu().numpy()) - lam).max() <= 1e-11 * np.abs(lam).max()
 
 
+# ---- the adjoint through the flow-rate controller ----
+#
+# The same transitional channel driven at constant flow rate: after every step
+# the controller sets the ambient force from the bulk velocity it reads, so
+# dJ/dG and dJ/dF^0 include the controller's response.  Each run restarts the
+# controller (set_ambient_force, then set_f) from the same force.
+
+
+def _flow_problem(precision, nx=32, ny=16, nz=8, T=24, **kw):
+    text = configs.adjoint_channel(nx=nx, ny=ny, nz=nz, flow_rate="auto", flow_axis=0, **kw)
+    eng = _engine(text, precision=precision)
+    G = 1e-6 * np.random.default_rng(99).standard_normal((3, nz, ny, nx))
+    return eng, eng.get_f(), eng.ambient_force(), G, T
+
+
+def _flow_start(eng, f0, F0, G):
+    eng.set_ambient_force(F0)
+    eng.set_f(f0)
+    eng.set_cell_force(*G)
+
+
+def _flow_grad(eng, f0, F0, G, T, kind=dugks_amd.LOSS_WALL_SHEAR, interval=0):
+    nx, ny, nz = eng.shape
+    _flow_start(eng, f0, F0, G)
+    gd = torch.empty((3, nz, ny, nx), dtype=torch.float64, device=torch.device("cuda"))
+    J = eng.forward_record(T, loss_kind=kind, interval=interval)
+    F_end = eng.ambient_force()
+    eng.adjoint_replay(gd)
+    torch.cuda.synchronize()
+    # the replay leaves the controller's force where the forward ended
+    assert (eng.ambient_force() == F_end).all()
+    return J, gd.cpu().numpy(), eng.sweep_force_gradient()
+
+
+def _flow_J(eng, f0, F0, G, T, kind=dugks_amd.LOSS_WALL_SHEAR):
+    _flow_start(eng, f0, F0, G)
+    return eng.forward_record(T, loss_kind=kind)
+
+
+@pytest.mark.parametrize("kind", [dugks_amd.LOSS_WALL_SHEAR, dugks_amd.LOSS_KINETIC_ENERGY],
+                         ids=["wall_shear", "kinetic_energy"])
+def test_flow_controlled_adjoint_matches_central_differences(rng, kind):
+    eng, f0, F0, G, T = _flow_problem(PREC_FP64)
+    J, grad, dF0 = _flow_grad(eng, f0, F0, G, T, kind)
+    # the forward moved the force (the controller is active)
+    assert eng.ambient_force()[0] != F0[0]
+    # recomputation from checkpoints with the recorded forces is exact
+    for interval in (1, 5, T):
+        Jk, gk, dk = _flow_grad(eng, f0, F0, G, T, kind, interval)
+        assert Jk == J and (gk == grad).all() and dk == dF0
+    for trial in range(4):
+        v = rng.standard_normal(G.shape) * 1e-6
+        fd = (_flow_J(eng, f0, F0, G + v, T, kind) - _flow_J(eng, f0, F0, G - v, T, kind)) / 2
+        an = float((grad * v).sum())
+        # measured 8e-10 (wall shear), 3e-9 (kinetic energy)
+        assert abs(fd - an) <= 1e-7 * abs(an), (trial, fd, an)
+    # the initial ambient force along the flow axis (measured 3e-8 / 5e-8,
+    # the O(d^2) truncation of the difference)
+    d = 1e-3 * abs(F0[0])
+    Fp, Fm = F0.copy(), F0.copy()
+    Fp[0] += d
+    Fm[0] -= d
+    fd = (_flow_J(eng, f0, Fp, G, T, kind) - _flow_J(eng, f0, Fm, G, T, kind)) / (2 * d)
+    assert abs(fd - dF0) <= 1e-6 * abs(dF0), (fd, dF0)
+
+
+def test_flow_controlled_gradient_differs_from_constant_force(rng):
+    # the controller's response is a large part of the gradient: the same
+    # problem at constant force gives a gradient far outside the 1e-6 bar
+    eng, f0, F0, G, T = _flow_problem(PREC_FP64)
+    _, g_flow, _ = _flow_grad(eng, f0, F0, G, T)
+    e2, f2, _, _ = _adjoint_problem(rng, PREC_FP64)
+    assert (f2 == f0).all()
+    _, g_const, _ = _grad(e2, f2, G, T)
+    # (measured 0.13)
+    assert np.abs(g_flow - g_const).max() > 1e-2 * np.abs(g_const).max()
+
+
+@pytest.mark.parametrize("precision", [PREC_FP32, PREC_FP16S], ids=["fp32", "fp16"])
+def test_flow_controlled_reduced_precision_gradient_against_fp64(rng, precision):
+    # FP32 arithmetic with lam in FP32 (the z-marching gather's compact
+    # records): against the FP64 sweep
+    e64, f64, F64, G, T = _flow_problem(PREC_FP64)
+    J64, g64, d64 = _flow_grad(e64, f64, F64, G, T)
+    e32, f32, F32, _, _ = _flow_problem(precision)
+    J32, g32, d32 = _flow_grad(e32, f32, F32, G, T)
+    # measured: FP32 1.1e-6, FP16 deviation storage 1.6e-5 (gradient)
+    tol = 1e-5 if precision == PREC_FP32 else 2e-4
+    assert abs(J32 - J64) <= tol * abs(J64)
+    assert np.abs(g32 - g64).max() <= tol * np.abs(g64).max()
+    assert abs(d32 - d64) <= tol * abs(d64)
+
+
+def test_flow_controlled_adjoint_unsupported_configurations():
+    # FP32 without the FP32-lam form (ny % 8 != 0), and streak forcing
+    eng, *_ = _flow_problem(PREC_FP32, ny=12)
+    with pytest.raises(dugks_amd.DugksError):
+        eng.forward_record(4)
+    eng, *_ = _flow_problem(PREC_FP64, perturb=1)
+    with pytest.raises(dugks_amd.DugksError):
+        eng.forward_record(4)
+    # no sweep with the controller replayed yet
+    eng, *_ = _flow_problem(PREC_FP64)
+    with pytest.raises(dugks_amd.DugksError):
+        eng.sweep_force_gradient()
+
+
 def test_autograd_function(rng):
     from dugks_amd.autograd import control_loss
     text = configs.adjoint_channel(nx=32, ny=16, nz=4)
