@@ -10,6 +10,8 @@
   controller.
 - Large channel, 256 x 192 x 256, 200 steps per mode: finite, mass conserved,
   bulk velocity held, FP32 against FP64.
+- Adjoint channel 64 x 64 x 32, T = 400, at constant force and at constant
+  flow rate: FP64 and FP32 gradients against central differences.
 
 Field errors are relative to the FP64 maxima (each profile to its own
 maximum), the definitions bench.py reports.
@@ -187,25 +189,31 @@ def test_large_channel_256x192x256():
         assert not bad, (name, err)
 
 
-def test_adjoint_channel_64x64x32_gradient():
+@pytest.mark.parametrize("flow", [False, True], ids=["constant_force", "constant_flow_rate"])
+def test_adjoint_channel_64x64x32_gradient(flow):
     # the adjoint workload at full size: transitional channel 64 x 64 x 32
     # (seeded perturbed laminar state, Re = 2000), J = time-mean wall shear
     # stress over T = 400 steps, control G ~ N(0, 1e-6) per cell, sqrt(T)
-    # checkpoints.  The FP64 gradient against central differences of J on
-    # 32 seeded random directions (bar 1e-4); the FP32 gradient against the
-    # FP64 one and against the same differences.
+    # checkpoints, driven by a constant force or at constant flow rate (the
+    # sweep then differentiates the controller; every run restarts it from
+    # the same force).  The FP64 gradient against central differences of J
+    # on 32 seeded random directions (bar 1e-4); the FP32 gradient against
+    # the FP64 one and against the same differences.
     torch = pytest.importorskip("torch")
     T, ndirs = 400, 32
     dev = torch.device("cuda", 0)
     grads, Js = {}, {}
     fds = []
+    kw = dict(flow_rate="auto", flow_axis=0) if flow else {}
     for name, prec in (("fp64", PREC_FP64), ("fp32", PREC_FP32)):
-        eng = _engine(configs.adjoint_channel(precision=name), prec)
+        eng = _engine(configs.adjoint_channel(precision=name, **kw), prec)
         nx, ny, nz = eng.shape
         f0 = torch.empty((19, nz, ny, nx), dtype=torch.float64, device=dev)
         eng.get_f_device(f0)
+        F0 = eng.ambient_force()
         G = torch.from_numpy(1e-6 * np.random.default_rng(2024).standard_normal((3, nz, ny, nx))).to(dev)
         gd = torch.empty_like(G)
+        eng.set_ambient_force(F0)
         eng.set_f_device(f0)
         eng.set_cell_force_device(G)
         Js[name], _ = eng.forward_adjoint(T, gd)
@@ -214,6 +222,7 @@ def test_adjoint_channel_64x64x32_gradient():
             rng = np.random.default_rng(7)
 
             def run_J(ctrl):
+                eng.set_ambient_force(F0)
                 eng.set_f_device(f0)
                 eng.set_cell_force_device(ctrl)
                 return eng.forward_record(T)
