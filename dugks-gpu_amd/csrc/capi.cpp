@@ -872,11 +872,19 @@ hipError_t pool_doubles(Ctx& c, double*& p, size_t& len, size_t n) {
   return hipSuccess;
 }
 
-// The step kernels leave the flow-rate controller alone while it lives.
+// While it lives, the step kernels leave the flow-rate controller alone;
+// when it ends (also on an error return) the ambient force along the flow
+// axis is set back to the one the recorded forward ended with, in stream
+// order.
 struct FrozenFlow {
   Ctx& c;
-  explicit FrozenFlow(Ctx& cx) : c(cx) { c.flow_frozen = true; }
-  ~FrozenFlow() { c.flow_frozen = false; }
+  int axis;
+  FrozenFlow(Ctx& cx, int ax) : c(cx), axis(ax) { c.flow_frozen = true; }
+  ~FrozenFlow() {
+    c.flow_frozen = false;
+    hipMemcpyAsync(c.famb_dev + axis, c.sweep.frec + c.sweep.T, sizeof(double), hipMemcpyDeviceToDevice,
+                   c.stream);
+  }
 };
 
 // Grow a list of state buffers to n entries (existing ones are kept).
@@ -1042,7 +1050,7 @@ int dugks_adjoint_replay(dugks_ctx* ctx, double* dev_grad_force, double* dev_gra
   };
   w.replayed = false;
   if (flow) {
-    frozen.reset(new FrozenFlow(c));
+    frozen.reset(new FrozenFlow(c, ax));
     if ((rc = chk(launch_fill(c, w.ctl, 0.0, kFlowCtlFlag0), "clear"))) return rc;
     if ((rc = chk(launch_fill(c, w.ctl + kFlowCtlLoss, 0.0, kFlowCtlLen - kFlowCtlLoss), "clear"))) return rc;
   }
@@ -1099,11 +1107,12 @@ int dugks_adjoint_replay(dugks_ctx* ctx, double* dev_grad_force, double* dev_gra
     }
   }
   if (flow) {
-    // nu_0 = phi_0 + nu_1 = dJ/dF^0, then the force the forward ended with
+    // nu_0 = phi_0 + nu_1 = dJ/dF^0; the force the forward ended with is set
+    // back as `frozen` ends, ahead of the copies below in stream order
     hipError_t e = launch_axis_sum(c, dev_grad_force, ax, w.ctl + kFlowCtlSumNew);
     if (e == hipSuccess) e = launch_flow_adjoint_scalar(c, w.ctl, w.fpart, nparts, true, invT);
-    if (e == hipSuccess) e = set_force(T);
     if ((rc = chk(e, "controller adjoint"))) return rc;
+    frozen.reset();
   }
   if (dev_grad_f0)
     if ((rc = chk(lam32 ? launch_widen(c, static_cast<const float*>(lam), dev_grad_f0, (long long)nf)

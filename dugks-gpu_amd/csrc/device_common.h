@@ -304,6 +304,10 @@ __device__ constexpr double wclass_weight(int k) { return k == 0 ? W0 : (k == 1 
 // feq_i = w_i rho (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u.u), held as the
 // per-weight-class products w rho (P - 3 Kf u.F) and 3 Kf w rho, so that
 // each population costs c.u, c.F and four fused multiply-adds.
+// Tag of the constructors that form the per-class products in FP64 and round
+// them to the arithmetic type once (see centre_fbar).
+struct ExactProducts {};
+
 template <class Real>
 struct EqTerm {
   Real ux, uy, uz, base, A[3], B[3];
@@ -317,6 +321,19 @@ struct EqTerm {
       const Real wr = Real(wclass_weight(k)) * rho;
       A[k] = wr * P0;
       B[k] = wr * K3;
+    }
+  }
+  __device__ __forceinline__ EqTerm(ExactProducts, Real rho, Real ux_, Real uy_, Real uz_, const Real F[3], Real P,
+                                    Real Kf)
+      : ux(ux_), uy(uy_), uz(uz_) {
+    base = Real(1) - Real(1.5) * (ux * ux + uy * uy + uz * uz);
+    const double uF = (double)ux * F[0] + (double)uy * F[1] + (double)uz * F[2];
+    const double P0 = (double)P - 3.0 * (double)Kf * uF;
+    const double K3 = 3.0 * (double)Kf;
+    for (int k = 0; k < 3; k++) {
+      const double wr = wclass_weight(k) * (double)rho;
+      A[k] = (Real)(wr * P0);
+      B[k] = (Real)(wr * K3);
     }
   }
   // G_ip given c_ip.F (cF, unused for the rest population)
@@ -393,6 +410,23 @@ struct EqShift {
       B[k] = w * b0;
     }
   }
+  __device__ __forceinline__ EqShift(ExactProducts, Real drho, Real ux_, Real uy_, Real uz_, const Real F[3], Real b,
+                                     Real Kf)
+      : ux(ux_), uy(uy_), uz(uz_) {
+    const double dr = (double)drho, rho = 1.0 + dr;
+    nuu = Real(-1.5) * (ux * ux + uy * uy + uz * uz);
+    const double uF = (double)ux * F[0] + (double)uy * F[1] + (double)uz * F[2];
+    const double K3 = 3.0 * (double)Kf;
+    const double c0 = (double)b * dr - K3 * rho * uF;
+    const double p0 = rho * ((double)b - K3 * uF);
+    const double b0 = K3 * rho;
+    for (int k = 0; k < 3; k++) {
+      const double w = wclass_weight(k);
+      C0[k] = (Real)(w * c0);
+      P[k] = (Real)(w * p0);
+      B[k] = (Real)(w * b0);
+    }
+  }
   // (1 - b) h + (G_ip - b w) given nb = -b, with h - b h one fused multiply-add
   template <int ip>
   __device__ __forceinline__ Real relax(Real nb, Real h, Real cF) const {
@@ -408,6 +442,43 @@ struct EqShift {
   }
 };
 
+// The equilibrium part of a cell's shifted populations under FP16
+// non-equilibrium storage: w_i rho eps_i + w_i drho, eps_i = 3 c.u +
+// 4.5 (c.u)^2 - 1.5 u.u, from the stored drho and u (both codecs directions
+// use it, so a stored cell decodes to what was encoded up to the FP16
+// rounding of s).
+template <class Real>
+struct NeqEq {
+  Real ux, uy, uz, nuu, Pw[3], Cw[3];
+  __device__ __forceinline__ NeqEq(Real drho, const Real (&u)[3]) : ux(u[0]), uy(u[1]), uz(u[2]) {
+    nuu = Real(-1.5) * (ux * ux + uy * uy + uz * uz);
+    const double dr = (double)drho;
+    for (int k = 0; k < 3; k++) {
+      Pw[k] = (Real)(wclass_weight(k) * (1.0 + dr));
+      Cw[k] = (Real)(wclass_weight(k) * dr);
+    }
+  }
+  template <int ip>
+  __device__ __forceinline__ Real eps() const {
+    if constexpr (ip == 0) {
+      return nuu;
+    } else {
+      const Real cu = cdot<ip>(ux, uy, uz);
+      return cu * (cu * Real(4.5) + Real(3)) + nuu;
+    }
+  }
+  // w_i rho eps_i + w_i drho
+  template <int ip>
+  __device__ __forceinline__ Real eq() const {
+    return fma_rn(Pw[wclass(ip)], eps<ip>(), Cw[wclass(ip)]);
+  }
+  // h_i from the stored s_i
+  template <int ip>
+  __device__ __forceinline__ Real shift_of(Real s) const {
+    return fma_rn(Pw[wclass(ip)], eps<ip>(), fma_rn(s, Real(wgt(ip) / (double)kNeqScale), Cw[wclass(ip)]));
+  }
+};
+
 // Centre stage in shifted populations: hb_i = f-bar+_i - w_i from h_i, the
 // cell's drho = rho - 1 and momentum velocity u0 (with FP16 non-equilibrium
 // storage the stored ones, which the decoded populations match to the FP16
@@ -415,7 +486,8 @@ struct EqShift {
 template <class Real>
 __device__ __forceinline__ void centre_fbar_shift(const Real (&h)[Q], Real drho, const Real (&u0)[3],
                                                   const Real F[3], Real hdt, Real bw0, Real bw2, Real (&hb)[Q]) {
-  const EqShift<Real> eq(drho, u0[0] + hdt * F[0], u0[1] + hdt * F[1], u0[2] + hdt * F[2], F, bw2, bw0);
+  const EqShift<Real> eq(ExactProducts{}, drho, u0[0] + hdt * F[0], u0[1] + hdt * F[1], u0[2] + hdt * F[2], F, bw2,
+                         bw0);
   static_for<Q>([&](auto I) {
     constexpr int ip = decltype(I)::value;
     hb[ip] = eq.template relax<ip>(-bw2, h[ip], cdot<ip>(F[0], F[1], F[2]));
@@ -435,10 +507,18 @@ __device__ __forceinline__ void centre_fbar_shift(const Real (&h)[Q], Real drho,
 // (the same value in exact arithmetic; rho must then be the sum of f), so
 // the stage keeps the mass to one unbiased rounding.  bw1 = 1 - bw2 enters
 // as f - bw2 f (EqTerm::relax_keep).
+// The per-class products w rho (bw2 - 3 bw0 u.F) and 3 bw0 w rho are formed
+// in FP64 and rounded once (ExactProducts): rounded twice in FP32 (w rho,
+// then times bw2) they took 0.9 % of the Re = 1600 Taylor-Green vortex's
+// dissipation away (0.22 % at Re = 400, 0.05 % at Re = 100: the same
+// amount in absolute terms), against 0.01 % rounded once.  Carrying the
+// weights to double-float precision, the equilibrium polynomial in FP64, or
+// f - bw2 f and the final sum in FP64 each left the shortfall as it was.
 template <class Real>
 __device__ __forceinline__ void centre_fbar(const Real (&f)[Q], Real rho, const Real (&u0)[3], const Real F[3],
                                             Real hdt, Real bw0, Real bw2, Real (&fb)[Q]) {
-  const EqTerm<Real> eq(rho, u0[0] + hdt * F[0], u0[1] + hdt * F[1], u0[2] + hdt * F[2], F, bw2, bw0);
+  const EqTerm<Real> eq(ExactProducts{}, rho, u0[0] + hdt * F[0], u0[1] + hdt * F[1], u0[2] + hdt * F[2], F, bw2,
+                        bw0);
   static_for<Q>([&](auto I) {
     constexpr int ip = decltype(I)::value;
     if constexpr (ip != 0 || sizeof(Real) != 4)
@@ -708,24 +788,21 @@ struct RawCell<Real, NeqHalf> {
     });
     neq_decode<Real>(rho, u0, s, ff);
   }
-  // shifted populations h_i = f_i - w_i = w_i (drho + rho eps_i + s_i / kNeqScale)
-  // straight from the stored drho, u and s (see EqShift)
+  // shifted populations h_i = f_i - w_i = w_i rho eps_i + w_i drho + w_i s_i / kNeqScale
+  // straight from the stored drho, u and s (see EqShift), with the
+  // per-class products w rho and w drho formed in FP64 and rounded once
+  // (NeqEq; as in the centre stage, FP32 products rounded twice bias the
+  // flow's dissipation)
   __device__ __forceinline__ void decode_shift(const KParams& P, Real (&h)[Q], Real& drho, Real (&u0)[3]) const {
     drho = (Real)m[0];
     u0[0] = (Real)m[1];
     u0[1] = (Real)m[2];
     u0[2] = (Real)m[3];
-    const Real rho = Real(1) + drho;
-    const Real nuu = Real(-1.5) * (u0[0] * u0[0] + u0[1] * u0[1] + u0[2] * u0[2]);
+    const NeqEq<Real> eq(drho, u0);
     static_for<Q>([&](auto I) {
       constexpr int ip = decltype(I)::value;
-      Real eps = nuu;
-      if constexpr (ip != 0) {
-        const Real cu = cdot<ip>(u0[0], u0[1], u0[2]);
-        eps = cu * (cu * Real(4.5) + Real(3)) + nuu;
-      }
       const Real sv = (Real)__half2float(__ushort_as_half((unsigned short)v[ip]));
-      h[ip] = Real(wgt(ip)) * fma_rn(sv, Real(1.0 / (double)kNeqScale), fma_rn(rho, eps, drho));
+      h[ip] = eq.template shift_of<ip>(sv);
     });
   }
 };
@@ -749,20 +826,14 @@ template <class Real>
 __device__ __forceinline__ void store_neq_shift(const StateBuf<NeqHalf>& b, unsigned cell, unsigned N,
                                                 const Real (&h)[Q], Real& drho, Real (&u0)[3]) {
   cell_rho_u<Real>(h, drho, u0, true);
-  const Real rho = Real(1) + drho;
-  const Real nuu = Real(-1.5) * (u0[0] * u0[0] + u0[1] * u0[1] + u0[2] * u0[2]);
+  const NeqEq<Real> eq(drho, u0);
   const unsigned vf = cell * 4u, vh = cell * 2u;
   b.putf((float)drho, vf, 0);
   for (int q = 0; q < 3; q++) b.putf((float)u0[q], vf, (q + 1) * 4u * N);
   static_for<Q>([&](auto I) {
     constexpr int ip = decltype(I)::value;
-    Real eps = nuu;
-    if constexpr (ip != 0) {
-      const Real cu = cdot<ip>(u0[0], u0[1], u0[2]);
-      eps = cu * (cu * Real(4.5) + Real(3)) + nuu;
-    }
-    const Real n = fma_rn(h[ip], Real(1.0 / wgt(ip)), -fma_rn(rho, eps, drho));
-    b.put(__float2half((float)(n * Real(kNeqScale))), vh, 16u * N + ip * 2u * N);
+    const Real n = (h[ip] - eq.template eq<ip>()) * Real((double)kNeqScale / wgt(ip));
+    b.put(__float2half((float)n), vh, 16u * N + ip * 2u * N);
   });
 }
 
