@@ -800,11 +800,13 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, vo
   if (!adjoint_faces_fused_covers(c) || (format != kFacesFull && c.precision == 0)) return hipSuccess;
   fused_supported(c, &tx, &ty);
   *handled = true;
-  // Compact records go with the z-marching gather (nx % 32 == 0): there the
-  // pass runs on 32 x 4 tiles, whose face rows are whole 128-byte lines
-  // (16-wide rows write half lines; 1.67 against 1.87 ms on the 256x192x256
-  // channel despite the larger halo)
-  const bool tall = tx == 16 && format == kFacesFull;
+  // Compact records with lam' in FP32 go with the z-marching gather (nx % 32
+  // == 0): there the pass runs on 32 x 4 tiles, whose face rows are whole
+  // 128-byte lines (16-wide rows write half lines; on the 256x192x256
+  // channel the adjoint step took 2.92 ms with 16 x 12 tiles against 2.76
+  // despite the smaller halo).  With lam' in FP64 the 16 x 12 tiles are the
+  // faster (3.30 against 3.47 ms).
+  const bool tall = tx == 16 && format != kFacesCompactLam32;
   if (fpart) {
     // with the ambient-force cotangent: the formats of the forward-adjoint
     // sweep (FP64, and FP32 arithmetic with lam in FP32)
@@ -847,7 +849,8 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, vo
 long long adjoint_faces_fused_blocks(const Ctx& c, int format) {
   int tx = 32, ty = 4, TZ = 0;
   if (!fused_supported(c, &tx, &ty)) return 0;
-  const dim3 g = tx == 16 && format == kFacesFull ? adj_faces_grid<16, 12>(c, &TZ) : adj_faces_grid<32, 4>(c, &TZ);
+  const dim3 g = tx == 16 && format != kFacesCompactLam32 ? adj_faces_grid<16, 12>(c, &TZ)
+                                                           : adj_faces_grid<32, 4>(c, &TZ);
   return (long long)g.x * g.y * g.z;
 }
 
