@@ -593,7 +593,12 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
         } else if constexpr (kNeq) {
           store_raw_cell<Real, Store>(bout, cell, (unsigned)P.N, P, fnew, true, rr, u0n);
         } else if (partial) {
-          cell_rho_u<Real>(fnew, rr, u0n, kShift);
+          // the controller's reading only: FP32 sums suffice (the moments
+          // that feed the step are summed in FP64, cell_rho_u)
+          Real jn[3];
+          cell_moments<Real>(fnew, rr, jn);
+          if constexpr (kShift) rr += Real(1);
+          moment_velocity<Real>(rr, jn, u0n);
         }
         // bulk velocity of the new state for the flow-rate controller
         // (per-cell moments in the arithmetic type, the sum over cells in FP64)
