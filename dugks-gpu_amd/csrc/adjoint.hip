@@ -350,10 +350,16 @@ constexpr int kGP = (kGY + 2) * kGW;  // staged positions per array and populati
 constexpr int kGT = kGX * kGY;        // lanes: one per cell column
 static_assert(kGP <= 2 * kGT, "two staged positions per lane at most");
 
-template <class Real, class Store, class Lam>
+// mom: the region of rho, j behind the face records that holds fin's (FP32
+// storage).  FLOW (FP32 storage, FP32 lam; a sweep through the flow-rate
+// controller): lam' still lacks the bulk velocity's cotangent ubar (dV/V)
+// (c_ia - u_a)/rho of f(n+1), whose rho, j the previous pass left in the
+// other region; it is added as lam' is used.
+template <class Real, class Store, class Lam, bool FLOW = false>
 __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __restrict__ fin,
                                                      const Lam* __restrict__ lam, const Real* __restrict__ faces,
-                                                     Lam* __restrict__ lam_prev, double* __restrict__ grad, int TZ) {
+                                                     Lam* __restrict__ lam_prev, double* __restrict__ grad, int TZ,
+                                                     int mom, const double* __restrict__ ubar, double inv_vol) {
   __shared__ Real S[3][Q][kGP];   // face cotangents of plane p: x-faces, y-faces, z-faces p+1/2
   __shared__ Real WXs[5][kGW];    // per staged column: wR for c = -1, 0, +1, then the shift weights
   __shared__ Real WYs[5][kGY + 2];  // per staged row
@@ -554,8 +560,14 @@ __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __r
   // with FP32 storage the face pass has stored rho and j of every cell
   // behind the records: 16 bytes read per cell instead of 76 of f
   constexpr bool kMoments = std::is_same<Store, float>::value;
-  const RawBuf mom(faces + 3LL * kCompact * P.eN, (unsigned)(4 * P.N * sizeof(Real)));
+  const RawBuf mb(faces + 3LL * kCompact * P.eN + mom * 4LL * P.N, (unsigned)(4 * P.N * sizeof(Real)));
   const unsigned mom_pop = (unsigned)(P.N * sizeof(Real));
+  // FLOW: rho and j_a of f(n+1) at the cell, and s0 = ubar dV / V
+  static_assert(!FLOW || kMoments, "the controller term needs the moments of f(n+1)");
+  const RawBuf mbn(faces + 3LL * kCompact * P.eN + (1 - mom) * 4LL * P.N, (unsigned)(4 * P.N * sizeof(Real)));
+  const int fa = P.flow_axis;
+  Real s0 = Real(0), rn = Real(1), jn = Real(0);
+  if constexpr (FLOW) s0 = (Real)(ubar[0] * P.wd[0][i0 + c + 1] * P.wd[1][j0 + r + 1] * P.wd[2][1] * inv_vol);
 
   // inputs of the centre stage of one cell: lam', rho and j, the cell force,
   // the gradient so far
@@ -567,10 +579,14 @@ __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __r
       lp[ip] = (Real)lamb.ld<Lam>(cell * (unsigned)sizeof(Lam), ip * lam_pop);
     });
     if constexpr (kMoments) {
-      rho = mom.ld<Real>(cell * (unsigned)sizeof(Real), 0);
-      j[0] = mom.ld<Real>(cell * (unsigned)sizeof(Real), mom_pop);
-      j[1] = mom.ld<Real>(cell * (unsigned)sizeof(Real), 2 * mom_pop);
-      j[2] = mom.ld<Real>(cell * (unsigned)sizeof(Real), 3 * mom_pop);
+      rho = mb.ld<Real>(cell * (unsigned)sizeof(Real), 0);
+      j[0] = mb.ld<Real>(cell * (unsigned)sizeof(Real), mom_pop);
+      j[1] = mb.ld<Real>(cell * (unsigned)sizeof(Real), 2 * mom_pop);
+      j[2] = mb.ld<Real>(cell * (unsigned)sizeof(Real), 3 * mom_pop);
+      if constexpr (FLOW) {
+        rn = mbn.ld<Real>(cell * (unsigned)sizeof(Real), 0);
+        jn = mbn.ld<Real>(cell * (unsigned)sizeof(Real), (unsigned)(1 + fa) * mom_pop);
+      }
     } else {
       Real ff[Q];
       static_for<Q>([&](auto I) {
@@ -629,6 +645,15 @@ __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __r
       const unsigned cell = (unsigned)(p - 1) * plane + colcell;
       const long long cl = (long long)cell;
       if constexpr (!kEarly) load_centre(cell);
+      if constexpr (FLOW) {
+        const Real inv = Real(1) / rn, sc = s0 * inv, t0 = -sc * (jn * inv);
+        const Real ex = fa == 0 ? Real(1) : Real(0), ey = fa == 1 ? Real(1) : Real(0),
+                   ez = fa == 2 ? Real(1) : Real(0);
+        static_for<Q>([&](auto I) {
+          constexpr int ip = decltype(I)::value;
+          lp[ip] += fma_rn(Real(CX[ip]) * ex + Real(CY[ip]) * ey + Real(CZ[ip]) * ez, sc, t0);
+        });
+      }
       static_for<Q>([&](auto I) {
         constexpr int ip = decltype(I)::value;
         accP[ip] += Real(4.0 / 3.0) * lp[ip];
@@ -687,7 +712,7 @@ int gather_z_chunk(const KParams& P) {
 // arithmetic with the z-marching gather only): lam' and lam in FP32.
 template <class Real, class Store, class Lam>
 hipError_t run_adjoint(Ctx& cx, const void* f, const Lam* lam_next, Lam* lam_prev, double* grad, int wall,
-                       double* fpart, long long fpart_len, long long* nparts) {
+                       double* fpart, long long fpart_len, long long* nparts, int mom, const double* ubar) {
   const KParams& P = cx.kp;
   Real* faces = static_cast<Real*>(cx.adj_faces);
   constexpr bool kLam32 = std::is_same<Lam, float>::value;
@@ -707,7 +732,11 @@ hipError_t run_adjoint(Ctx& cx, const void* f, const Lam* lam_next, Lam* lam_pre
     cx.adj_faces_format = fmt;
   }
   const int format = !compact ? kFacesFull : (kLam32 ? kFacesCompactLam32 : kFacesCompact);
-  hipError_t e = launch_adjoint_faces_fused(cx, f, lam_next, faces, format, &handled, fpart, fpart_len, nparts);
+  // the controller term rides on lam' only on the FP32-storage, FP32-lam form
+  constexpr bool kFuse = std::is_same<Store, float>::value && kLam32;
+  if (ubar && !kFuse) return hipErrorNotSupported;
+  hipError_t e = launch_adjoint_faces_fused(cx, f, lam_next, faces, format, &handled, fpart, fpart_len, nparts, mom,
+                                            ubar);
   if (e != hipSuccess) return e;
   // the force cotangent of the face collisions comes from the fused pass only
   if (fpart && !handled) return hipErrorNotSupported;
@@ -715,9 +744,18 @@ hipError_t run_adjoint(Ctx& cx, const void* f, const Lam* lam_next, Lam* lam_pre
     if (compact) {
       const int TZ = gather_z_chunk(P);
       dim3 grid(P.n[0] / kGX, P.n[1] / kGY, (P.n[2] + TZ - 1) / TZ);
+      const double inv_vol = 1.0 / (cx.s.width[0] * cx.s.width[1] * cx.s.width[2]);
+      if constexpr (kFuse) {
+        if (ubar) {
+          hipLaunchKernelGGL((k_adj_cells_z<Real, Store, Lam, true>), grid, dim3(kGT), 0, cx.stream, P,
+                             static_cast<const Store*>(f), lam_next, static_cast<const Real*>(cx.adj_faces),
+                             lam_prev, grad, TZ, mom, ubar, inv_vol);
+          return hipGetLastError();
+        }
+      }
       hipLaunchKernelGGL((k_adj_cells_z<Real, Store, Lam>), grid, dim3(kGT), 0, cx.stream, P,
                          static_cast<const Store*>(f), lam_next, static_cast<const Real*>(cx.adj_faces), lam_prev,
-                         grad, TZ);
+                         grad, TZ, mom, nullptr, inv_vol);
       return hipGetLastError();
     }
   }
@@ -757,7 +795,7 @@ namespace {
 
 template <class Lam>
 hipError_t adjoint_step(Ctx& c, const void* f, const Lam* lam_next, Lam* lam_prev, double* grad_force,
-                        double* fpart, long long fpart_len, long long* nparts) {
+                        double* fpart, long long fpart_len, long long* nparts, int mom, const double* ubar) {
   const Setup& s = c.s;
   if (s.recon != 0) return hipErrorNotSupported;
   int walls = 0, wall = -1;
@@ -783,13 +821,15 @@ hipError_t adjoint_step(Ctx& c, const void* f, const Lam* lam_next, Lam* lam_pre
     case 0:
       if constexpr (std::is_same<Lam, double>::value)
         return run_adjoint<double, double, Lam>(c, f, lam_next, lam_prev, grad_force, wall, fpart, fpart_len,
-                                                nparts);
+                                                nparts, mom, ubar);
       else
         return hipErrorNotSupported;
     case 1:
-      return run_adjoint<float, float, Lam>(c, f, lam_next, lam_prev, grad_force, wall, fpart, fpart_len, nparts);
+      return run_adjoint<float, float, Lam>(c, f, lam_next, lam_prev, grad_force, wall, fpart, fpart_len, nparts,
+                                            mom, ubar);
     default:
-      return run_adjoint<float, __half, Lam>(c, f, lam_next, lam_prev, grad_force, wall, fpart, fpart_len, nparts);
+      return run_adjoint<float, __half, Lam>(c, f, lam_next, lam_prev, grad_force, wall, fpart, fpart_len, nparts,
+                                             mom, ubar);
   }
 }
 
@@ -797,13 +837,16 @@ hipError_t adjoint_step(Ctx& c, const void* f, const Lam* lam_next, Lam* lam_pre
 
 hipError_t launch_adjoint(Ctx& c, const void* f, const double* lam_next, double* lam_prev, double* grad_force,
                           double* fpart, long long fpart_len, long long* nparts) {
-  return adjoint_step<double>(c, f, lam_next, lam_prev, grad_force, fpart, fpart_len, nparts);
+  return adjoint_step<double>(c, f, lam_next, lam_prev, grad_force, fpart, fpart_len, nparts, 0, nullptr);
 }
 
 hipError_t launch_adjoint_lam32(Ctx& c, const void* f, const float* lam_next, float* lam_prev, double* grad_force,
-                                double* fpart, long long fpart_len, long long* nparts) {
-  return adjoint_step<float>(c, f, lam_next, lam_prev, grad_force, fpart, fpart_len, nparts);
+                                double* fpart, long long fpart_len, long long* nparts, int mom,
+                                const double* ubar) {
+  return adjoint_step<float>(c, f, lam_next, lam_prev, grad_force, fpart, fpart_len, nparts, mom, ubar);
 }
+
+bool adjoint_flow_fused(const Ctx& c) { return c.precision == 1 && adjoint_lam32_supported(c); }
 
 bool adjoint_lam32_supported(const Ctx& c) {
   if (c.precision == 0 || c.precision == 3 || c.s.recon != 0) return false;

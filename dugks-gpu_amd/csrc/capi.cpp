@@ -1042,6 +1042,7 @@ int dugks_adjoint_replay(dugks_ctx* ctx, double* dev_grad_force, double* dev_gra
   // step also yields the face part of the force cotangent phi_n, and the
   // change of the summed cell force gradient gives the rest.
   const bool flow = w.flow;
+  const bool fused = flow && lam32 && adjoint_flow_fused(c);
   const int ax = c.s.stream_axis();
   std::unique_ptr<FrozenFlow> frozen;
   long long nparts = 0;
@@ -1084,13 +1085,18 @@ int dugks_adjoint_replay(dugks_ctx* ctx, double* dev_grad_force, double* dev_gra
                                   false, ax);
         if (e == hipSuccess)
           e = launch_flow_adjoint_scalar(c, w.ctl, w.fpart, n < T - 1 ? nparts : 0, n == 0, invT);
-        if (e == hipSuccess)
+        // the bulk-velocity term: added to lam as the adjoint step reads it
+        // where the step holds rho, j of f^{n+1} (from the step n+1 before
+        // it, in the other of the two moment regions), else a pass of its own
+        const bool ride = fused && n < T - 1;
+        if (e == hipSuccess && !ride)
           e = lam32 ? launch_flow_cotangent_lam32(c, state(n + 1), w.ctl + kFlowCtlUbar, static_cast<float*>(lam))
                     : launch_flow_cotangent(c, state(n + 1), w.ctl + kFlowCtlUbar, static_cast<double*>(lam));
         if (e == hipSuccess) e = set_force(n);
         if (e == hipSuccess)
           e = lam32 ? launch_adjoint_lam32(c, state(n), static_cast<const float*>(lam), static_cast<float*>(lam2),
-                                           dev_grad_force, w.fpart, (long long)w.fpart_len, &nparts)
+                                           dev_grad_force, w.fpart, (long long)w.fpart_len, &nparts,
+                                           fused ? (n & 1) : 0, ride ? w.ctl + kFlowCtlUbar : nullptr)
                     : launch_adjoint(c, state(n), static_cast<const double*>(lam), static_cast<double*>(lam2),
                                      dev_grad_force, w.fpart, (long long)w.fpart_len, &nparts);
         if ((rc = chk(e, "controller adjoint"))) return rc;

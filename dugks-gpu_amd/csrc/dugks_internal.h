@@ -71,9 +71,13 @@ constexpr int kFacesFull = 0, kFacesCompact = 1, kFacesCompactLam32 = 2;
 // values, at most fpart_len, else hipErrorInvalidValue); only FP64 full
 // records and FP32-lam compact records have this form, else
 // hipErrorNotSupported.
+// mom (compact records, FP32 storage): which of the two rho, j regions
+// behind the records the pass writes; ubar (FP32-lam sweep through the
+// flow-rate controller, with fpart): the bulk-velocity cotangent to add to
+// lam' from the other region (see AdjFaceIO).
 hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, void* faces, int format,
                                       bool* handled, double* fpart = nullptr, long long fpart_len = 0,
-                                      long long* nparts = nullptr);
+                                      long long* nparts = nullptr, int mom = 0, const double* ubar = nullptr);
 // Whether launch_adjoint_faces_fused covers the configuration.
 bool adjoint_faces_fused_covers(const Ctx& c);
 // Number of per-block force partials the pass writes for a format.
@@ -102,9 +106,14 @@ hipError_t launch_adjoint(Ctx& c, const void* f, const double* lam_next, double*
                           long long* nparts = nullptr);
 // The same with lam' and lam in FP32; only FP32 arithmetic on the z-marching
 // gather's configurations (adjoint_lam32_supported), else hipErrorNotSupported.
+// mom, ubar: as launch_adjoint_faces_fused; with ubar the gather adds the
+// same controller term (FP32 storage only: adjoint_flow_fused).
 hipError_t launch_adjoint_lam32(Ctx& c, const void* f, const float* lam_next, float* lam_prev,
                                 double* grad_force, double* fpart = nullptr, long long fpart_len = 0,
-                                long long* nparts = nullptr);
+                                long long* nparts = nullptr, int mom = 0, const double* ubar = nullptr);
+// Whether the sweep's adjoint step adds the flow-rate controller's term to
+// lam' as it reads it (FP32 storage with FP32 lam) instead of a pass of its own.
+bool adjoint_flow_fused(const Ctx& c);
 bool adjoint_lam32_supported(const Ctx& c);
 // Whether the adjoint covers the flow-rate controller (the ambient-force
 // cotangent of the face collisions is available).

@@ -44,7 +44,15 @@ struct Tile {
 struct AdjFaceIO {
   const void* lam;    // lam' = dJ/df(n+1), SoA [19][nz][ny][nx], FP64 or (LamT = float) FP32
   void* faces;        // out: [3][19][ghost-inclusive face slot], arithmetic type; COMPACT:
-                      // [3][kCompact][slot], then (FP32 storage) rho, j of every cell [4][N]
+                      // [3][kCompact][slot], then (FP32 storage) two [4][N] regions of
+                      // rho, j of every cell
+  int mom = 0;        // COMPACT, FP32 storage: the region the pass writes rho, j of fin into
+  // FP32 sweep through the flow-rate controller: lam' still lacks the bulk
+  // velocity's cotangent ubar (dV/V) (c_ia - u_a)/rho of f(n+1), whose rho, j
+  // the previous pass left in the other region; added as lam' is loaded
+  // (nullptr: lam' is complete)
+  const double* ubar = nullptr;
+  double inv_vol = 0;  // 1 / V
 };
 
 // ADJ = false: one DUGKS step fin -> fout.
@@ -235,8 +243,49 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
       if constexpr (ip != 0) l[ip] = (Real)lamb.ld<LamT>(vb, ip * lam_pop);
     });
   };
-  Real lamc[Q], lamn[Q];
-  if constexpr (ADJ) load_lam(z0, lamc);
+  // the flow-rate controller's term of lam' (AdjFaceIO::ubar): rho and j_a
+  // of f(n+1) at the column's cell of plane k, loaded with lam', added before
+  // lam' is used: lam'_i += s (c_ia - u_a), s = ubar (dV/V) / rho
+  constexpr bool kFlowFuse = ADJ && FADJ && COMPACT && std::is_same<Store, float>::value;
+  const bool fuse = kFlowFuse && adj.ubar != nullptr;
+  const RawBuf momn(static_cast<Real*>(adj.faces) + 3LL * kCompact * P.eN + (1 - adj.mom) * 4LL * P.N,
+                    (unsigned)(4 * P.N * sizeof(Real)));
+  Real s0 = Real(0), ex = Real(0), ey = Real(0), ez = Real(0);
+  if constexpr (kFlowFuse) {
+    if (fuse && lam_cell)
+      s0 = (Real)(adj.ubar[0] * P.wd[0][mi] * P.wd[1][mj] * P.wd[2][1] * adj.inv_vol);
+    ex = as == 0 ? Real(1) : Real(0);
+    ey = as == 1 ? Real(1) : Real(0);
+    ez = as == 2 ? Real(1) : Real(0);
+  }
+  auto load_flow = [&](int k, Real& rn, Real& jn) {
+    if constexpr (kFlowFuse) {
+      const unsigned vb = fuse && lam_cell ? (unsigned)(((long long)k * ny + gj) * nx + gi) * sizeof(Real)
+                                           : kOutOfRange;
+      rn = momn.ld<Real>(vb, 0);
+      jn = momn.ld<Real>(vb, (unsigned)((1 + as) * P.N * sizeof(Real)));
+    }
+  };
+  auto add_flow = [&](Real (&l)[Q], Real rn, Real jn) {
+    if constexpr (kFlowFuse) {
+      // rn is 0 where nothing was loaded (s0 is 0 there)
+      const Real inv = Real(1) / (rn != Real(0) ? rn : Real(1));
+      const Real s = s0 * inv, t0 = -s * (jn * inv);
+      static_for<Q>([&](auto I) {
+        constexpr int ip = decltype(I)::value;
+        if constexpr (ip != 0) l[ip] += fma_rn(Real(CX[ip]) * ex + Real(CY[ip]) * ey + Real(CZ[ip]) * ez, s, t0);
+      });
+    }
+  };
+  Real lamc[Q], lamn[Q], rnn = Real(0), jnn = Real(0);
+  if constexpr (ADJ) {
+    load_lam(z0, lamc);
+    if constexpr (kFlowFuse) {
+      Real rn = Real(0), jn = Real(0);
+      load_flow(z0, rn, jn);
+      add_flow(lamc, rn, jn);
+    }
+  }
   // compact form (COMPACT): per face the kCompact values of compact_record
   auto store_compact = [&](int a, long long slot, const Real (&rec)[kCompact]) {
     const RawBuf fb(static_cast<Real*>(adj.faces) + (long long)a * kCompact * P.eN, kCompact * face_pop);
@@ -334,7 +383,8 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
   // A chunk writes its planes z0+1 .. zend (mod nz), so the chunks cover
   // every plane once.
   auto store_moments = [&](int k, Real rho, const Real (&jm)[3]) {
-    const RawBuf mb(static_cast<Real*>(adj.faces) + 3LL * kCompact * P.eN, (unsigned)(4 * P.N * sizeof(Real)));
+    const RawBuf mb(static_cast<Real*>(adj.faces) + 3LL * kCompact * P.eN + adj.mom * 4LL * P.N,
+                    (unsigned)(4 * P.N * sizeof(Real)));
     const unsigned vb = (((unsigned)k * ny + gj) * nx + gi) * (unsigned)sizeof(Real);
     const unsigned pop = (unsigned)(P.N * sizeof(Real));
     mb.st(rho, vb, 0);
@@ -349,7 +399,10 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
   auto plane = [&](int p, Real (&fbc)[Q], Real (&fbn)[Q], Real (&gc)[Q], Real (&gn)[Q], Real (&lamc)[Q],
                    Real (&lamn)[Q]) {
     const int kn = (p + 1) % nz;
-    if constexpr (ADJ) load_lam(kn, lamn);
+    if constexpr (ADJ) {
+      load_lam(kn, lamn);
+      load_flow(kn, rnn, jnn);
+    }
     if (own) {
       if constexpr (!kPrefetch) load_own(kn, fpre);
       Real F[3];
@@ -515,6 +568,8 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
       }
     }
     Real zp[10];
+    // lam' of plane p+1 complete (its loads were issued at the top of the plane)
+    if constexpr (kFlowFuse) add_flow(lamn, rnn, jnn);
     if constexpr (ADJ) {
       if (interior) {
         // +z face of the cell: between planes p and p+1 (z is periodic)
@@ -729,7 +784,8 @@ dim3 adj_faces_grid(const Ctx& cx, int* TZ) {
 // FADJ: also the per-block force partials into fpart (nparts of them).
 template <class Real, class Store, int TX, int TY, bool COMPACT = false, class LamT = double, bool FADJ = false>
 hipError_t run_adj_faces(Ctx& cx, const void* f, const void* lam, void* faces, double* fpart = nullptr,
-                         long long fpart_len = 0, long long* nparts = nullptr) {
+                         long long fpart_len = 0, long long* nparts = nullptr, int mom = 0,
+                         const double* ubar = nullptr) {
   const KParams& P = cx.kp;
   using T = Tile<TX, TY>;
   int TZ = 0;
@@ -741,7 +797,7 @@ hipError_t run_adj_faces(Ctx& cx, const void* f, const void* lam, void* faces, d
   }
   hipLaunchKernelGGL((k_fused<Real, Store, TX, TY, true, COMPACT, LamT, FADJ>), grid, dim3(T::NT), 0, cx.stream, P,
                      static_cast<const Store*>(f), static_cast<Store*>(nullptr), TZ, FADJ ? fpart : nullptr,
-                     AdjFaceIO{lam, faces});
+                     AdjFaceIO{lam, faces, mom, ubar, 1.0 / (cx.s.width[0] * cx.s.width[1] * cx.s.width[2])});
   return hipGetLastError();
 }
 
@@ -799,19 +855,21 @@ bool adjoint_faces_fused_covers(const Ctx& c) {
 }
 
 hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, void* faces, int format,
-                                      bool* handled, double* fpart, long long fpart_len, long long* nparts) {
+                                      bool* handled, double* fpart, long long fpart_len, long long* nparts, int mom,
+                                      const double* ubar) {
   int tx = 32, ty = 4;
   *handled = false;
   if (!adjoint_faces_fused_covers(c) || (format != kFacesFull && c.precision == 0)) return hipSuccess;
   fused_supported(c, &tx, &ty);
   *handled = true;
-  // Compact records with lam' in FP32 go with the z-marching gather (nx % 32
-  // == 0): there the pass runs on 32 x 4 tiles, whose face rows are whole
-  // 128-byte lines (16-wide rows write half lines; on the 256x192x256
-  // channel the adjoint step took 2.92 ms with 16 x 12 tiles against 2.76
-  // despite the smaller halo).  With lam' in FP64 the 16 x 12 tiles are the
-  // faster (3.30 against 3.47 ms).
-  const bool tall = tx == 16 && format != kFacesCompactLam32;
+  // Compact records go with the z-marching gather (nx % 32 == 0): there the
+  // pass runs on 32 x 4 tiles, whose face rows are whole 128-byte lines
+  // (16-wide rows write half lines; on the 256x192x256 channel the adjoint
+  // step with FP32 lam took 2.92 ms with 16 x 12 tiles against 2.76 despite
+  // the smaller halo).  With FP64 lam 16 x 12 tiles were faster (3.30
+  // against 3.47 ms), but both lam forms keep the same tiles: the same
+  // kernel code then gives the same bits for a lam' that FP32 holds exactly.
+  const bool tall = tx == 16 && format == kFacesFull;
   if (fpart) {
     // with the ambient-force cotangent: the formats of the forward-adjoint
     // sweep (FP64, and FP32 arithmetic with lam in FP32)
@@ -821,7 +879,8 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, vo
                   : run_adj_faces<double, double, 32, 4, false, double, true>(c, f, lam, faces, fpart, fpart_len,
                                                                                nparts);
     if (c.precision == 1 && format == kFacesCompactLam32 && !tall)
-      return run_adj_faces<float, float, 32, 4, true, float, true>(c, f, lam, faces, fpart, fpart_len, nparts);
+      return run_adj_faces<float, float, 32, 4, true, float, true>(c, f, lam, faces, fpart, fpart_len, nparts, mom,
+                                                                   ubar);
     if (c.precision == 2 && format == kFacesCompactLam32 && !tall)
       return run_adj_faces<float, __half, 32, 4, true, float, true>(c, f, lam, faces, fpart, fpart_len, nparts);
     return hipErrorNotSupported;
@@ -832,11 +891,11 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, vo
                   : run_adj_faces<double, double, 32, 4>(c, f, lam, faces);
     case 1:
       if (format == kFacesCompactLam32)
-        return tall ? run_adj_faces<float, float, 16, 12, true, float>(c, f, lam, faces)
-                    : run_adj_faces<float, float, 32, 4, true, float>(c, f, lam, faces);
+        return tall ? run_adj_faces<float, float, 16, 12, true, float>(c, f, lam, faces, nullptr, 0, nullptr, mom)
+                    : run_adj_faces<float, float, 32, 4, true, float>(c, f, lam, faces, nullptr, 0, nullptr, mom);
       if (format == kFacesCompact)
-        return tall ? run_adj_faces<float, float, 16, 12, true>(c, f, lam, faces)
-                    : run_adj_faces<float, float, 32, 4, true>(c, f, lam, faces);
+        return tall ? run_adj_faces<float, float, 16, 12, true>(c, f, lam, faces, nullptr, 0, nullptr, mom)
+                    : run_adj_faces<float, float, 32, 4, true>(c, f, lam, faces, nullptr, 0, nullptr, mom);
       return tall ? run_adj_faces<float, float, 16, 12>(c, f, lam, faces)
                   : run_adj_faces<float, float, 32, 4>(c, f, lam, faces);
     default:
@@ -854,8 +913,7 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, vo
 long long adjoint_faces_fused_blocks(const Ctx& c, int format) {
   int tx = 32, ty = 4, TZ = 0;
   if (!fused_supported(c, &tx, &ty)) return 0;
-  const dim3 g = tx == 16 && format != kFacesCompactLam32 ? adj_faces_grid<16, 12>(c, &TZ)
-                                                           : adj_faces_grid<32, 4>(c, &TZ);
+  const dim3 g = tx == 16 && format == kFacesFull ? adj_faces_grid<16, 12>(c, &TZ) : adj_faces_grid<32, 4>(c, &TZ);
   return (long long)g.x * g.y * g.z;
 }
 
