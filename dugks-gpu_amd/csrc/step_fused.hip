@@ -20,6 +20,11 @@
 // Two barriers per plane: [f-bar(p+1); write Z(p), Zn(p+1/2)] | [faces] |
 // [update], the update of one plane overlapping the next plane's f-bar.
 // z must be periodic; x and y may be periodic or walls, uniform or stretched.
+// GRAD: the gradient reconstruction instead (calc_fx, src/dugks.cu:421-583:
+// the mean of the two cells beside the face less h c.g, with the normal
+// derivative from those two cells and the tangential ones by central
+// differences of the two-cell means), on uniform axes; the LDS then holds a
+// ring of the f-bar+ planes p-1, p and p+1, written as each is computed.
 // Other configurations use the two-pass kernels.
 #include "device_common.h"
 
@@ -55,6 +60,27 @@ struct AdjFaceIO {
   double inv_vol = 0;  // 1 / V
 };
 
+// Gradient reconstruction of one population at a face's foot point: the
+// mean m of the two cells beside the face less h c.g (calc_fx,
+// src/dugks.cu:421-583); the terms of the components c_a = 0 are left out
+// (they add zero).
+template <int cx, int cy, int cz, class Real>
+__device__ __forceinline__ Real grad_foot(Real m, Real h, Real gx, Real gy, Real gz) {
+  if constexpr (cx == 0 && cy == 0 && cz == 0) {
+    return m;
+  } else {
+    Real s;
+    if constexpr (cx != 0) s = Real(cx) * gx;
+    if constexpr (cy != 0) {
+      if constexpr (cx != 0) s += Real(cy) * gy; else s = Real(cy) * gy;
+    }
+    if constexpr (cz != 0) {
+      if constexpr (cx != 0 || cy != 0) s += Real(cz) * gz; else s = Real(cz) * gz;
+    }
+    return m - h * s;
+  }
+}
+
 // ADJ = false: one DUGKS step fin -> fout.
 // ADJ = true: the adjoint-face pass of the step taken from fin: every face is
 // reconstructed and its collision linearised as in the step, and the
@@ -63,7 +89,7 @@ struct AdjFaceIO {
 // cotangent of the ambient force along the flow axis through the face
 // collisions, summed per block into partial[tile] (fixed order).
 template <class Real, class Store, int TX, int TY, bool ADJ = false, bool COMPACT = false, class LamT = double,
-          bool FADJ = false>
+          bool FADJ = false, bool GRAD = false>
 // FP32 arithmetic: two blocks per CU (at most 256 registers per lane)
 __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64, sizeof(Real) == 4 ? 2 : 1) k_fused(
     KParams P, const Store* __restrict__ fin,
@@ -71,6 +97,7 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
                                                           double* __restrict__ partial, AdjFaceIO adj) {
   using T = Tile<TX, TY>;
   constexpr int AC = T::AC;
+  static_assert(!(GRAD && ADJ), "the adjoint differentiates the trilinear reconstruction");
   // With FP16 non-equilibrium storage the step carries shifted populations
   // h = f - w (see EqShift): f-bar+, the LDS planes, the face values and the
   // update are all f - w, decoded from and encoded to the stored rho - 1, u
@@ -84,8 +111,10 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
   auto ia = [](int rr, int cc) { return CM ? cc * (TY + 2) + rr : rr * AC + cc; };       // A, An, Lam
   auto ic = [](int rr, int cc) { return CM ? cc * TY + rr : rr * (TX + 1) + cc; };       // Cx (rows 0..TY-1)
   auto id = [](int rr, int cc) { return CM ? cc * (TY + 1) + rr : rr * TX + cc; };       // Dy (rows 0..TY)
-  __shared__ Real A[Q][(TY + 2) * AC];        // Z(p): tangential z-lerp of f-bar(p)
-  __shared__ Real An[Q][(TY + 2) * AC];       // Zn(p+1/2): normal z-lerp between p and p+1
+  // GRAD: A holds the ring of f-bar+ planes, plane k in slot (k + 3) % 3
+  // (population ip of slot s at A[s * Q + ip]); An is unused
+  __shared__ Real A[GRAD ? 3 * Q : Q][(TY + 2) * AC];  // Z(p): tangential z-lerp of f-bar(p)
+  __shared__ Real An[GRAD ? 1 : Q][(TY + 2) * AC];     // Zn(p+1/2): normal z-lerp between p and p+1
   __shared__ Real Cx[ADJ ? 1 : 10][TY * (TX + 1)];  // crossing populations of -x faces
   __shared__ Real Dy[ADJ ? 1 : 10][(TY + 1) * TX];  // crossing populations of -y faces
   __shared__ Real Lam[ADJ ? Q : 1][(TY + 2) * AC];  // adjoint-face pass: lam'(p) of the tile and halo
@@ -152,6 +181,15 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
   const Real wzt_p = tz(9 + 7);  // tangential, toward p-1 (c_z = +1)
   const Real wzn[3] = {tz(9 + 0), tz(9 + 1), tz(9 + 2)};  // normal, toward p+1
   const Real updz = tz(9 + 8);
+  // GRAD: 1/d and 1/(2d) per axis (every axis uniform)
+  Real gin[3] = {Real(0), Real(0), Real(0)}, gtn[3] = {Real(0), Real(0), Real(0)};
+  if constexpr (GRAD) {
+    for (int a = 0; a < 3; a++) {
+      gin[a] = (Real)(1.0 / P.du[a]);
+      gtn[a] = (Real)(0.5 / P.du[a]);
+    }
+  }
+  auto rslot = [](int k) { return (k + 3) % 3; };  // ring slot of plane k >= -1
   // Per-column and per-row weights of the tile, copied once into LDS: the
   // tables are not written by this kernel, but the compiler cannot prove that
   // against the f stores and would reload them (or hold them in registers)
@@ -215,9 +253,64 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
       fc.template put<ip>(v);
     });
   };
-  auto zface = [&](Real (&out)[10]) {
+  // GRAD: population ip of ring slot sl at LDS position (rr, cc)
+  auto ga = [&](int sl, int ip, int rr, int cc) -> Real { return A[sl * Q + ip][ia(rr, cc)]; };
+  // GRAD reconstructions from the ring (slots sm, s0, sp of planes p-1, p,
+  // p+1): the x-face between tile columns c-1 and c of row r, the y-face
+  // between rows r-1 and r of column c, both at plane p, and the z-face
+  // between planes p (s0) and p+1 (sp) of the thread's column
+  auto grad_face_x = [&](Face<Real, 0>& fc, int sm, int s0, int sp) {
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      constexpr int cx = CX[ip], cy = CY[ip], cz = CZ[ip];
+      const Real fl = ga(s0, ip, r, c), fr = ga(s0, ip, r, c + 1);
+      Real gy = Real(0), gz = Real(0);
+      if constexpr (cy != 0)
+        gy = (Real(0.5) * (ga(s0, ip, r + 1, c) + ga(s0, ip, r + 1, c + 1)) -
+              Real(0.5) * (ga(s0, ip, r - 1, c) + ga(s0, ip, r - 1, c + 1))) * gtn[1];
+      if constexpr (cz != 0)
+        gz = (Real(0.5) * (ga(sp, ip, r, c) + ga(sp, ip, r, c + 1)) -
+              Real(0.5) * (ga(sm, ip, r, c) + ga(sm, ip, r, c + 1))) * gtn[2];
+      fc.template put<ip>(grad_foot<cx, cy, cz>(Real(0.5) * (fr + fl), hdt, (fr - fl) * gin[0], gy, gz));
+    });
+  };
+  auto grad_face_y = [&](Face<Real, 1>& fc, int sm, int s0, int sp) {
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      constexpr int cx = CX[ip], cy = CY[ip], cz = CZ[ip];
+      const Real fl = ga(s0, ip, r - 1, c + 1), fr = ga(s0, ip, r, c + 1);
+      Real gx = Real(0), gz = Real(0);
+      if constexpr (cx != 0)
+        gx = (Real(0.5) * (ga(s0, ip, r - 1, c + 2) + ga(s0, ip, r, c + 2)) -
+              Real(0.5) * (ga(s0, ip, r - 1, c) + ga(s0, ip, r, c))) * gtn[0];
+      if constexpr (cz != 0)
+        gz = (Real(0.5) * (ga(sp, ip, r - 1, c + 1) + ga(sp, ip, r, c + 1)) -
+              Real(0.5) * (ga(sm, ip, r - 1, c + 1) + ga(sm, ip, r, c + 1))) * gtn[2];
+      fc.template put<ip>(grad_foot<cx, cy, cz>(Real(0.5) * (fr + fl), hdt, gx, (fr - fl) * gin[1], gz));
+    });
+  };
+  auto grad_face_z = [&](Face<Real, 2>& fc, int s0, int sp) {
+    static_for<Q>([&](auto I) {
+      constexpr int ip = decltype(I)::value;
+      constexpr int cx = CX[ip], cy = CY[ip], cz = CZ[ip];
+      const Real fl = ga(s0, ip, r, c + 1), fr = ga(sp, ip, r, c + 1);
+      Real gx = Real(0), gy = Real(0);
+      if constexpr (cx != 0)
+        gx = (Real(0.5) * (ga(s0, ip, r, c + 2) + ga(sp, ip, r, c + 2)) -
+              Real(0.5) * (ga(s0, ip, r, c) + ga(sp, ip, r, c))) * gtn[0];
+      if constexpr (cy != 0)
+        gy = (Real(0.5) * (ga(s0, ip, r + 1, c + 1) + ga(sp, ip, r + 1, c + 1)) -
+              Real(0.5) * (ga(s0, ip, r - 1, c + 1) + ga(sp, ip, r - 1, c + 1))) * gtn[1];
+      fc.template put<ip>(grad_foot<cx, cy, cz>(Real(0.5) * (fr + fl), hdt, gx, gy, (fr - fl) * gin[2]));
+    });
+  };
+  // the +z face of plane p (GRAD: from ring slots s0 and sp)
+  auto zface = [&](Real (&out)[10], int s0, int sp) {
     Face<Real, 2> fc;
-    zface_rec(fc);
+    if constexpr (GRAD)
+      grad_face_z(fc, s0, sp);
+    else
+      zface_rec(fc);
     if constexpr (kShift)
       fc.collide_shift(Famb, hdt, fw0, fw2, out);
     else
@@ -369,10 +462,20 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
         fbm[k] = fbn[ZP[k]];
       });
     }
-    store_zn(fbn, fbc);
+    if constexpr (GRAD) {
+      if (active) {
+        static_for<Q>([&](auto I) {
+          constexpr int ip = decltype(I)::value;
+          A[rslot(z0 - 1) * Q + ip][ia(r, c + 1)] = fbn[ip];
+          A[rslot(z0) * Q + ip][ia(r, c + 1)] = fbc[ip];
+        });
+      }
+    } else {
+      store_zn(fbn, fbc);
+    }
     __syncthreads();
     if constexpr (!ADJ) {
-      if (interior) zface(zm);
+      if (interior) zface(zm, rslot(z0 - 1), rslot(z0));
     }
     __syncthreads();
   }
@@ -430,7 +533,16 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
       fbar_column<Real, Store, kShift>(P, bin, Nb, gi, gj, kn, Famb, fbn);
     }
     // ---- phase 1: Z(p) (tangential z-lerp of f-bar(p)) and Zn(p+1/2) ----
-    if (active) {
+    // (GRAD: f-bar+(p+1) into its ring slot)
+    const int sm = rslot(p - 1), s0 = rslot(p), sp = rslot(p + 1);
+    if constexpr (GRAD) {
+      if (active) {
+        static_for<Q>([&](auto I) {
+          constexpr int ip = decltype(I)::value;
+          A[sp * Q + ip][ia(r, c + 1)] = fbn[ip];
+        });
+      }
+    } else if (active) {
       static_for<Q>([&](auto I) {
         constexpr int ip = decltype(I)::value;
         constexpr int cz = CZ[ip];
@@ -444,7 +556,7 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
         }
       });
     }
-    store_zn(fbc, fbn);
+    if constexpr (!GRAD) store_zn(fbc, fbn);
     if constexpr (ADJ) {
       if (active) {
         static_for<Q>([&](auto I) {
@@ -463,7 +575,8 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
     if (do_x) {
       // face between x-columns gi-1 (left, ghost-inclusive index mi-1) and gi
       Face<Real, 0> fc;
-      static_for<Q>([&](auto I) {
+      if constexpr (GRAD) grad_face_x(fc, sm, s0, sp);
+      else static_for<Q>([&](auto I) {
         constexpr int ip = decltype(I)::value;
         constexpr int cx = CX[ip], cy = CY[ip];
         if constexpr (ip == 0) {
@@ -518,7 +631,8 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
     if (do_y) {
       // face between y-rows gj-1 (below) and gj
       Face<Real, 1> fc;
-      static_for<Q>([&](auto I) {
+      if constexpr (GRAD) grad_face_y(fc, sm, s0, sp);
+      else static_for<Q>([&](auto I) {
         constexpr int ip = decltype(I)::value;
         constexpr int cx = CX[ip], cy = CY[ip];
         if constexpr (ip == 0) {
@@ -593,7 +707,7 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
         }
       }
     } else {
-      if (interior) zface(zp);
+      if (interior) zface(zp, s0, sp);
     }
     __syncthreads();
     // ---- phase 3: update of plane p ----
@@ -662,14 +776,15 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
         fpre.settle();
       }
     }
-    if (active) {
+    if (!GRAD && active) {
       static_for<5>([&](auto K) {
         constexpr int k = decltype(K)::value;
         fbm[k] = fbc[ZP[k]];
       });
     }
-    // no barrier: the next phase 1 writes A/An, which phase 3 does not read,
-    // and the next phase 2 writes Cx/Dy only after the next phase-1 barrier
+    // no barrier: the next phase 1 writes A/An (GRAD: the ring slot of p-1,
+    // last read in this plane's phase 2), which phase 3 does not read, and
+    // the next phase 2 writes Cx/Dy only after the next phase-1 barrier
   };
   // The step runs planes in pairs; the adjoint-face pass (more registers
   // live across the loop) one plane per iteration with the values of p+1
@@ -747,7 +862,7 @@ int z_chunk(const Ctx& cx, int TX, int TY) {
   return tz;
 }
 
-template <class Real, class Store, int TX, int TY>
+template <class Real, class Store, int TX, int TY, bool GRAD>
 hipError_t run_fused(Ctx& cx, const void* fin, void* fout) {
   const KParams& P = cx.kp;
   using T = Tile<TX, TY>;
@@ -761,7 +876,8 @@ hipError_t run_fused(Ctx& cx, const void* fin, void* fout) {
     partial = cx.flow_partial;
   }
   if (cx.timing) hipEventRecord(cx.tev[0], cx.stream);
-  hipLaunchKernelGGL((k_fused<Real, Store, TX, TY>), grid, dim3(T::NT), 0, cx.stream, P,
+  hipLaunchKernelGGL((k_fused<Real, Store, TX, TY, false, false, double, false, GRAD>), grid, dim3(T::NT), 0,
+                     cx.stream, P,
                      static_cast<const Store*>(fin), static_cast<Store*>(fout), TZ, partial, AdjFaceIO{});
   if (cx.timing) hipEventRecord(cx.tev[1], cx.stream);
   if (flow) {
@@ -803,9 +919,11 @@ hipError_t run_adj_faces(Ctx& cx, const void* f, const void* lam, void* faces, d
 
 }  // namespace
 
+// Either reconstruction (gradient reconstruction needs uniform axes, which
+// the configuration enforces); the adjoint-face mode trilinear only.
 bool fused_supported(const Ctx& c, int* tx, int* ty) {
   const Setup& s = c.s;
-  if (s.recon != 0 || !s.periodic[2] || !s.uniform[2]) return false;
+  if (!s.periodic[2] || !s.uniform[2]) return false;
   // buffer offsets are 32-bit: one state array must span < 4 GiB
   if ((double)c.state_bytes >= 4294967296.0) return false;
   // Tiles of 4 waves: 16 x 12 cells when the grid divides (1.31 f-bar
@@ -827,28 +945,40 @@ bool fused_supported(const Ctx& c, int* tx, int* ty) {
   return true;
 }
 
+namespace {
+template <bool GRAD>
+hipError_t launch_fused_recon(Ctx& c, const void* fin, void* fout, bool tall) {
+  switch (c.precision) {
+    case 0:
+      return tall ? run_fused<double, double, 16, 12, GRAD>(c, fin, fout)
+                  : run_fused<double, double, 32, 4, GRAD>(c, fin, fout);
+    case 1:
+      return tall ? run_fused<float, float, 16, 12, GRAD>(c, fin, fout)
+                  : run_fused<float, float, 32, 4, GRAD>(c, fin, fout);
+    case 2:
+      return tall ? run_fused<float, __half, 16, 12, GRAD>(c, fin, fout)
+                  : run_fused<float, __half, 32, 4, GRAD>(c, fin, fout);
+    default:
+      return tall ? run_fused<float, NeqHalf, 16, 12, GRAD>(c, fin, fout)
+                  : run_fused<float, NeqHalf, 32, 4, GRAD>(c, fin, fout);
+  }
+}
+}  // namespace
+
 hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled) {
   int tx = 32, ty = 4;
   *handled = false;
   if (!fused_supported(c, &tx, &ty)) return hipSuccess;
   *handled = true;
-  const bool tall = tx == 16;
-  switch (c.precision) {
-    case 0:
-      return tall ? run_fused<double, double, 16, 12>(c, fin, fout) : run_fused<double, double, 32, 4>(c, fin, fout);
-    case 1:
-      return tall ? run_fused<float, float, 16, 12>(c, fin, fout) : run_fused<float, float, 32, 4>(c, fin, fout);
-    case 2:
-      return tall ? run_fused<float, __half, 16, 12>(c, fin, fout) : run_fused<float, __half, 32, 4>(c, fin, fout);
-    default:
-      return tall ? run_fused<float, NeqHalf, 16, 12>(c, fin, fout) : run_fused<float, NeqHalf, 32, 4>(c, fin, fout);
-  }
+  return c.s.recon == 0 ? launch_fused_recon<false>(c, fin, fout, tx == 16)
+                        : launch_fused_recon<true>(c, fin, fout, tx == 16);
 }
 
 bool adjoint_faces_fused_covers(const Ctx& c) {
   int tx = 32, ty = 4;
-  // the adjoint does not cover FP16 non-equilibrium storage
-  if (c.kernel == 0 || c.precision == 3 || !fused_supported(c, &tx, &ty)) return false;
+  // the adjoint does not cover FP16 non-equilibrium storage or the gradient
+  // reconstruction
+  if (c.kernel == 0 || c.precision == 3 || c.s.recon != 0 || !fused_supported(c, &tx, &ty)) return false;
   // lam' and each face array are addressed with 32-bit buffer offsets
   const double rb = c.precision == 0 ? 8.0 : 4.0;
   return (double)Q * c.kp.N * 8.0 < 2147483648.0 && (double)Q * c.kp.eN * rb < 2147483648.0;

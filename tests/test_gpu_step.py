@@ -71,11 +71,32 @@ CASES = {
     "tall_tiles": dict(nx=64, ny=24, nz=40, periods_y=0, ambient_fx=1e-4, ambient_fz=-1e-4, xwidth=6.4,
                        ywidth=2.4, zwidth=4.0),
 }
+# gradient reconstruction (calc_fx; uniform axes): both kernels, the fused
+# one on 32 x 4 and 16 x 12 tiles, walls on x, y or both, several tiles
+# with XCD ordering and z chunks (nz = 40 and 80 take 8-plane chunks), and
+# nz = 3, where the ring's planes p - 1 and p + 1 are one plane apart
+GRAD_CASES = {
+    "grad_periodic": dict(nx=32, ny=8, nz=5, recon="grad", ambient_fx=1e-4, xwidth=3.2, ywidth=0.8, zwidth=0.5),
+    "grad_walls_x": dict(nx=32, ny=8, nz=4, periods_x=0, recon="grad", ambient_fz=2e-4, xwidth=3.2, ywidth=0.8,
+                         zwidth=0.4),
+    "grad_walls_y": dict(nx=32, ny=16, nz=4, periods_y=0, recon="grad", ambient_fx=2e-4, xwidth=3.2, ywidth=1.6,
+                         zwidth=0.4),
+    "grad_walls_xy": dict(nx=32, ny=16, nz=3, periods_x=0, periods_y=0, recon="grad", ambient_fz=1e-4,
+                          xwidth=3.2, ywidth=1.6, zwidth=0.3),
+    "grad_tall_walls_y": dict(nx=32, ny=24, nz=6, periods_y=0, recon="grad", ambient_fx=2e-4, xwidth=3.2,
+                              ywidth=2.4, zwidth=0.6),
+    "grad_tall_walls_x": dict(nx=16, ny=12, nz=5, periods_x=0, recon="grad", ambient_fz=2e-4, xwidth=1.6,
+                              ywidth=1.2, zwidth=0.5),
+    "grad_tall_tiles": dict(nx=64, ny=24, nz=40, periods_y=0, recon="grad", ambient_fx=1e-4, ambient_fz=-1e-4,
+                            xwidth=6.4, ywidth=2.4, zwidth=4.0),
+    "grad_tiles_xcd": dict(nx=64, ny=16, nz=80, periods_y=0, recon="grad", ambient_fx=2e-4, xwidth=6.4,
+                           ywidth=1.6, zwidth=8.0),
+}
 # configurations only the two-pass kernels cover
 TWOPASS_CASES = {
     "ragged": dict(nx=13, ny=6, nz=5, xwidth=1.3, ywidth=0.6, zwidth=0.5),
     "walls_z": dict(nx=6, ny=4, nz=10, periods_z=0, ambient_fy=2e-4, xwidth=0.6, ywidth=0.4, zwidth=1.0),
-    "grad": dict(nx=8, ny=6, nz=5, recon="grad", ambient_fx=1e-4, xwidth=0.8, ywidth=0.6, zwidth=0.5),
+    "grad_ragged": dict(nx=8, ny=6, nz=5, recon="grad", ambient_fx=1e-4, xwidth=0.8, ywidth=0.6, zwidth=0.5),
 }
 
 
@@ -99,6 +120,17 @@ def _one_step_parity(rng, case, kernel):
 @pytest.mark.parametrize("case", list(CASES))
 def test_fp64_one_step_matches_oracle(rng, case, kernel):
     _one_step_parity(rng, CASES[case], kernel)
+
+
+@pytest.mark.parametrize("kernel", KERNELS, ids=["twopass", "fused"])
+@pytest.mark.parametrize("case", list(GRAD_CASES))
+def test_fp64_gradient_reconstruction_matches_oracle(rng, case, kernel):
+    _one_step_parity(rng, GRAD_CASES[case], kernel)
+
+
+def test_auto_selects_fused_kernel_for_gradient_reconstruction():
+    eng = dugks_amd.Engine(configs.base(visc=0.02, **GRAD_CASES["grad_walls_y"]))
+    assert eng.time_steps(2)["launches_per_step"] == 1
 
 
 @pytest.mark.parametrize("case", list(TWOPASS_CASES))
@@ -145,9 +177,10 @@ def test_cell_force_matches_oracle(rng, kernel):
 #   rho and u stay FP32: at most 2^-11 |f - feq|.
 @pytest.mark.parametrize("precision,tol", [(PREC_FP32, 2e-6), (PREC_FP16S, None), (PREC_FP16N, None)],
                          ids=["fp32", "fp16", "fp16n"])
-def test_reduced_precision_one_step(rng, precision, tol, kernel, nx):
+@pytest.mark.parametrize("recon", ["tri", "grad"])
+def test_reduced_precision_one_step(rng, precision, tol, kernel, nx, recon):
     text = configs.base(nx=nx, ny=16, nz=4, periods_y=0, visc=0.02, ambient_fx=2e-4,
-                        xwidth=nx / 10, ywidth=1.6, zwidth=0.4)
+                        xwidth=nx / 10, ywidth=1.6, zwidth=0.4, recon=recon)
     eng = _engine(text, precision=precision, kernel=kernel)
     ora = _oracle_for(eng, text)
     f0 = random_state(eng.shape, rng)
