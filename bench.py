@@ -381,6 +381,46 @@ def study_channel(dugks_amd, configs, torch, np, steps):
     return out
 
 
+def study_recon(dugks_amd, configs, torch, np, steps):
+    """Trilinear against gradient reconstruction on the 256 x 192 x 256
+    Re_tau = 180 channel with a uniform wall-normal mesh (the gradient form
+    needs uniform axes), on the fused and the two-pass kernels: time per
+    step over `steps` steps from the synthetic state, the largest relative
+    difference between the FP64 fused and two-pass states after them, and
+    the FP32 and FP16 non-equilibrium field errors against FP64."""
+    prec = {"fp64": dugks_amd.PREC_FP64, "fp32": dugks_amd.PREC_FP32, "fp16n": dugks_amd.PREC_FP16N}
+    kern = {"fused": dugks_amd.KERNEL_FUSED, "twopass": dugks_amd.KERNEL_TWOPASS}
+    out = {"workload": "Re_tau=180 channel 256x192x256, uniform y, constant flow rate", "steps": steps,
+           "runs": {}, "fused_vs_twopass_fp64": {}, "errors_vs_fp64": {}}
+    for recon in ("tri", "grad"):
+        stats, f64 = {}, {}
+        for m in ("fp64", "fp32", "fp16n"):
+            for k in ("fused", "twopass"):
+                if k == "twopass" and m == "fp16n":
+                    continue
+                text = configs.large_channel(precision=m, nsteps=steps, mesh_y="uniform", recon=recon)
+                eng = dugks_amd.Engine(text, precision=prec[m], kernel=kern[k], device=0,
+                                       stream=torch.cuda.current_stream().cuda_stream)
+                t = eng.time_steps(steps)
+                ms = t["total_ms"] / steps
+                out["runs"][f"{recon}/{m}/{k}"] = {"ms_per_step": round(ms, 4),
+                                                   "glups": round(eng.ncell / ms / 1e6, 3)}
+                log(f"recon {recon} {m} {k}: {ms:.3f} ms per step, {eng.ncell / ms / 1e6:.2f} GLUPS")
+                if m == "fp64":
+                    f64[k] = torch.empty(19 * eng.ncell, dtype=torch.float64, device="cuda")
+                    eng.get_f_device(f64[k])
+                if k == "fused":
+                    stats[m] = field_stats(eng, np)
+                eng.close()
+        d = (f64["fused"] - f64["twopass"]).abs().max() / f64["twopass"].abs().max()
+        out["fused_vs_twopass_fp64"][recon] = float(f"{float(d):.3e}")
+        f64.clear()
+        out["errors_vs_fp64"][recon] = {m: {k: float(f"{v:.3e}") for k, v in
+                                            field_errors(stats["fp64"], stats[m], np).items()}
+                                        for m in ("fp32", "fp16n")}
+    return out
+
+
 def study_adjoint(dugks_amd, configs, torch, np, T, ndirs=32, flow=False):
     """Adjoint workload: transitional channel 64x64x32 (seeded perturbed
     laminar state, Re = 2000), J = time-mean wall shear stress over T steps,
@@ -475,11 +515,13 @@ def main():
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--adj-steps", type=int, default=1000,
                     help="steps of the forward+adjoint sweep on the large channel (0: skip)")
-    ap.add_argument("--study", choices=["tgv", "channel", "adjoint", "adjoint_flow"],
+    ap.add_argument("--study", choices=["tgv", "channel", "adjoint", "adjoint_flow", "recon"],
                     help="instead of the benchmark: the reduced-precision error study on the "
-                         "Taylor-Green vortex or the 128^3 channel, or the adjoint gradient study "
-                         "at constant force or constant flow rate (prints one JSON line)")
-    ap.add_argument("--study-steps", type=int, default=0, help="steps of the study (default 1000 / 10000)")
+                         "Taylor-Green vortex or the 128^3 channel, the adjoint gradient study "
+                         "at constant force or constant flow rate, or trilinear against gradient "
+                         "reconstruction on the uniform 256x192x256 channel (prints one JSON line)")
+    ap.add_argument("--study-steps", type=int, default=0,
+                    help="steps of the study (default 1000 / 10000 / 400 / 100)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB read + 1 GiB write elementwise kernel 5 times "
                          "(the reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -513,6 +555,8 @@ def main():
         import numpy as np
         if args.study == "tgv":
             res = study_tgv(dugks_amd, configs, torch, np, args.study_steps or 1000)
+        elif args.study == "recon":
+            res = study_recon(dugks_amd, configs, torch, np, args.study_steps or 100)
         elif args.study in ("adjoint", "adjoint_flow"):
             res = study_adjoint(dugks_amd, configs, torch, np, args.study_steps or 400,
                                 flow=args.study == "adjoint_flow")
