@@ -412,6 +412,13 @@ MIRROR_CASES = {
                         ambient_fz=2e-4, xwidth=2.0, ywidth=0.8, zwidth=0.4)),
     "periodic_z": (2, dict(nx=32, ny=24, nz=5, ambient_fx=1e-4, ambient_fz=-2e-4, xwidth=3.2, ywidth=2.4,
                            zwidth=0.5)),
+    # gradient reconstruction: the signs of its tangential differences
+    "grad_walls_y_tall": (1, dict(nx=32, ny=24, nz=6, periods_y=0, recon="grad", ambient_fx=2e-4,
+                                  ambient_fz=1e-4, xwidth=3.2, ywidth=2.4, zwidth=0.6)),
+    "grad_walls_x": (0, dict(nx=32, ny=8, nz=4, periods_x=0, recon="grad", ambient_fy=1e-4, ambient_fz=2e-4,
+                             xwidth=3.2, ywidth=0.8, zwidth=0.4)),
+    "grad_periodic_z": (2, dict(nx=32, ny=24, nz=5, recon="grad", ambient_fx=1e-4, ambient_fz=-2e-4, xwidth=3.2,
+                                ywidth=2.4, zwidth=0.5)),
 }
 
 
