@@ -848,17 +848,22 @@ __global__ void __launch_bounds__(256) k_flow_finish(const double* partial, int 
   }
 }
 
-// z-chunk length: 64 planes (fewer re-computed chunk boundaries) unless that
-// leaves fewer than 8 tiles per CU (256 CUs, two resident blocks each: four
-// rounds of blocks, so the last round's uneven finish costs little); then
-// halve it, down to 8.  On the 256x192x256 channel (256 columns of 16x12)
-// that is 32 planes: FP32 0.823 against 0.833 ms with 64, FP16 deviation
-// storage 0.738 against 0.758 ms (16: 0.833 / 0.740; 128: 0.865 / 0.783).
+// z-chunk length: 32 planes unless that leaves fewer tiles than one round of
+// two resident blocks on each of the 256 CUs; then halve it, down to 8.  A
+// chunk recomputes f-bar+ of the two planes before it, so short chunks cost
+// that much more work; more rounds of blocks even out the finish.  On the
+// 256x192x256 channel (256 columns of 16x12, 2048 tiles): FP32 0.823 ms
+// against 0.833 with 64 planes and 0.833 with 16, FP16 deviation storage
+// 0.738 against 0.758 and 0.740 (128 planes: 0.865 / 0.783).  On 128^3
+// (128 columns of 32x4, 512 tiles) 32 planes against the 8 that a rule of
+// eight tiles per CU gives: Taylor-Green FP32 0.148 against 0.166 ms, FP64
+// 0.397 against 0.444 ms; the Re_tau = 180 channel FP32 0.185 against 0.195,
+// FP64 0.426 against 0.472 ms.
 int z_chunk(const Ctx& cx, int TX, int TY) {
   if (cx.fused_tz > 0) return cx.fused_tz;
   const long long cols = (long long)(cx.kp.n[0] / TX) * (cx.kp.n[1] / TY);
-  int tz = 64;
-  while (tz > 8 && cols * ((cx.kp.n[2] + tz - 1) / tz) < 8 * 256) tz /= 2;
+  int tz = 32;
+  while (tz > 8 && cols * ((cx.kp.n[2] + tz - 1) / tz) < 2 * 256) tz /= 2;
   return tz;
 }
 
@@ -928,9 +933,10 @@ bool fused_supported(const Ctx& c, int* tx, int* ty) {
   if ((double)c.state_bytes >= 4294967296.0) return false;
   // Tiles of 4 waves: 16 x 12 cells when the grid divides (1.31 f-bar
   // evaluations per cell, the x-halo lanes packed behind the rows), else
-  // 32 x 4 (2.0 per cell).  Four waves per block keep FP64 at one wave per
-  // SIMD without spills and fit two FP32 blocks (about 200 registers per
-  // lane) on a CU.
+  // 32 x 4 (1.59 per cell, the last wave holding only the 12 x-halo lanes;
+  // 16 x 8 tiles of 3 waves were no faster on 128^3).  Four waves per block
+  // keep FP64 at one wave per SIMD without spills and fit two FP32 blocks
+  // (at most 256 registers per lane) on a CU.
   if (s.n[0] % 16 == 0 && s.n[1] % 12 == 0) {
     *tx = 16;
     *ty = 12;
