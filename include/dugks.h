@@ -46,7 +46,11 @@ extern "C" {
 /* Step kernels. */
 #define DUGKS_KERNEL_AUTO (-1)
 #define DUGKS_KERNEL_TWOPASS 0 /* f-bar kernel + ghost fill + face/update kernel */
-#define DUGKS_KERNEL_FUSED 1   /* one fused kernel per step */
+#define DUGKS_KERNEL_FUSED 1   /* one fused kernel per step: z periodic and uniform, and nx % 16 == 0,
+                                  ny % 12 == 0 or nx % 32 == 0, ny % 4 == 0, one state array
+                                  < 4 GiB; either reconstruction
+                                  (config key recon = tri | grad); DUGKS_KERNEL_AUTO takes it
+                                  whenever these hold */
 
 typedef struct dugks_ctx dugks_ctx;
 
