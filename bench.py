@@ -175,7 +175,7 @@ def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32"):
     lam_type = "float32" if "float32" in t_adj else "float64"
     S = {"fp64": 8, "fp32": 4}[precision]
     lb = 4 if lam_type == "float32" else 8
-    rec = 2 * 3 * 13 * 4 if precision != "fp64" else 2 * 3 * 19 * 8
+    rec = 2 * 3 * 13 * S  # compact records (the z-marching gather's configurations)
     # FP32 storage: the face pass hands rho and j (16 B) to the gather, which
     # then does not read f
     state = 19 * S + 2 * 16 if precision == "fp32" else 2 * 19 * S
@@ -214,7 +214,7 @@ def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32"):
                sweep_device_in_use_bytes=peak, sweep_pool_bytes=pool,
                device_free_before=free0, device_total=total,
                finite=bool(ok))
-    log(f"adjoint: step {t_adj[lam_type]*1e3:.2f} ms ({lam_type} lam; {t_adj['float64']*1e3:.2f} ms with float64) "
+    log(f"adjoint {precision}: step {t_adj[lam_type]*1e3:.2f} ms ({lam_type} lam; {t_adj['float64']*1e3:.2f} ms with float64) "
         f"vs forward {fwd_step*1e3:.3f} ms; "
         f"{T}-step sweep {t_fa:.2f} s ({t_fa/t_fwd:.1f}x forward), pool {pool/1e9:.1f} GB, "
         f"{peak/1e9:.1f} GB in use")
@@ -515,6 +515,9 @@ def main():
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--adj-steps", type=int, default=1000,
                     help="steps of the forward+adjoint sweep on the large channel (0: skip)")
+    ap.add_argument("--adj-modes", default="fp32,fp64",
+                    help="arithmetic of the adjoint legs (the first is reported as 'adjoint', the "
+                         "others as 'adjoint_<mode>')")
     ap.add_argument("--study", choices=["tgv", "channel", "adjoint", "adjoint_flow", "recon"],
                     help="instead of the benchmark: the reduced-precision error study on the "
                          "Taylor-Green vortex or the 128^3 channel, the adjoint gradient study "
@@ -581,9 +584,14 @@ def main():
                 log(f"{m} vs fp64 after {nstep} steps: u max {errors[m]['u_max']:.2e}, "
                     f"U profile {errors[m]['U_profile_max']:.2e}")
     stats.clear()
-    adjoint = None
+    adjoint, adjoint_more = None, {}
     if args.adj_steps > 0:
-        adjoint = adjoint_leg(dugks_amd, configs, torch, args.adj_steps)
+        for i, am in enumerate(m for m in args.adj_modes.split(",") if m):
+            res_a = adjoint_leg(dugks_amd, configs, torch, args.adj_steps, precision=am)
+            if i == 0:
+                adjoint = res_a
+            else:
+                adjoint_more["adjoint_" + am] = res_a
     primary = "fp32" if "fp32" in results else modes[0]
     r = results[primary]
     traffic = measured_traffic(primary)
@@ -620,6 +628,7 @@ def main():
         "cpu_baseline": cpu,
         "precision_errors": errors,
         "adjoint": adjoint,
+        **adjoint_more,
         "modes": {m: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res.items()}
                   for m, res in results.items()},
     }

@@ -17,7 +17,7 @@
 //      reconstructed values (in the arithmetic type);
 //   3. the gather: S3^T of those values over the faces whose stencil
 //      contains each cell (the wall ghost extrapolation folded in), then S5^T
-//      and S1^T.  k_adj_cells_z (FP32 arithmetic, 32 x 8 column tiles)
+//      and S1^T.  k_adj_cells_z (32 x 8 column tiles)
 //      stages each plane of face cotangents in LDS and marches along z;
 //      k_adj_cells (every other case) runs one thread per cell over face
 //      arrays with periodic ghost copies (k_face_ghosts).
@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(256) k_adj_cells(KParams P, const Store* __res
   }
 }
 
-// ---- kernel 3, z-marching form (FP32 arithmetic, 32 x 8 column tiles) ----
+// ---- kernel 3, z-marching form (32 x 8 column tiles) ----
 // A workgroup owns a kGX x kGY tile of cell columns and marches along z over
 // a chunk of planes.  Per plane it stages the x-, y- and z-face cotangents of
 // the tile and a one-deep ring ((kGY+2) x (kGX+2) ghost-inclusive positions,
@@ -687,17 +687,18 @@ __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __r
   }
 }
 
-// The z-marching gather covers FP32 arithmetic on 32 x 8 column tiles with z
-// periodic and uniform, at most one wall axis (x or y), and face arrays that
-// 32-bit buffer offsets reach.
+// The z-marching gather covers FP32 and FP64 arithmetic on 32 x 8 column
+// tiles with z periodic and uniform, at most one wall axis (x or y), and face
+// arrays and lam' that 32-bit buffer offsets reach.
 bool gather_z_supported(const Ctx& c) {
   const Setup& s = c.s;
-  if (c.precision == 0 || c.kernel == 0) return false;
+  if (c.kernel == 0) return false;
   if (s.n[0] % kGX != 0 || s.n[1] % kGY != 0 || !s.periodic[2] || !s.uniform[2]) return false;
   if (!s.periodic[0] && !s.periodic[1]) return false;
   if (!s.periodic[0] && s.n[0] < 3) return false;
   if (!s.periodic[1] && s.n[1] < 3) return false;
-  if ((double)Q * c.kp.eN * 4.0 >= 2147483648.0 || (double)Q * c.kp.N * 8.0 >= 2147483648.0) return false;
+  const double rb = c.precision == 0 ? 8.0 : 4.0;
+  if ((double)Q * c.kp.eN * rb >= 2147483648.0 || (double)Q * c.kp.N * 8.0 >= 2147483648.0) return false;
   return true;
 }
 
@@ -717,11 +718,10 @@ hipError_t run_adjoint(Ctx& cx, const void* f, const Lam* lam_next, Lam* lam_pre
   Real* faces = static_cast<Real*>(cx.adj_faces);
   constexpr bool kLam32 = std::is_same<Lam, float>::value;
   bool handled = false;
-  // FP32 arithmetic with the z-marching gather: the fused adjoint-face pass
-  // writes the compact face records, which the gather expands
-  bool compact = false;
-  if constexpr (!std::is_same<Real, double>::value)
-    compact = gather_z_supported(cx) && adjoint_faces_fused_covers(cx);
+  // With the z-marching gather the fused adjoint-face pass writes the
+  // compact face records, which the gather expands
+  const int format = adjoint_record_format(cx, kLam32);
+  const bool compact = format != kFacesFull;
   if (kLam32 && !compact) return hipErrorNotSupported;
   // Each format leaves some slots unwritten (ghost rows, missing faces) that
   // must read as 0: clear the arrays when the format changes.
@@ -731,7 +731,6 @@ hipError_t run_adjoint(Ctx& cx, const void* f, const Lam* lam_next, Lam* lam_pre
     if (e0 != hipSuccess) return e0;
     cx.adj_faces_format = fmt;
   }
-  const int format = !compact ? kFacesFull : (kLam32 ? kFacesCompactLam32 : kFacesCompact);
   // the controller term rides on lam' only on the FP32-storage, FP32-lam form
   constexpr bool kFuse = std::is_same<Store, float>::value && kLam32;
   if (ubar && !kFuse) return hipErrorNotSupported;
@@ -740,7 +739,7 @@ hipError_t run_adjoint(Ctx& cx, const void* f, const Lam* lam_next, Lam* lam_pre
   if (e != hipSuccess) return e;
   // the force cotangent of the face collisions comes from the fused pass only
   if (fpart && !handled) return hipErrorNotSupported;
-  if constexpr (!std::is_same<Real, double>::value) {
+  {
     if (compact) {
       const int TZ = gather_z_chunk(P);
       dim3 grid(P.n[0] / kGX, P.n[1] / kGY, (P.n[2] + TZ - 1) / TZ);
@@ -844,6 +843,14 @@ hipError_t launch_adjoint_lam32(Ctx& c, const void* f, const float* lam_next, fl
                                 double* fpart, long long fpart_len, long long* nparts, int mom,
                                 const double* ubar) {
   return adjoint_step<float>(c, f, lam_next, lam_prev, grad_force, fpart, fpart_len, nparts, mom, ubar);
+}
+
+// The face-record format of an adjoint step: compact records for the
+// z-marching gather wherever it and the fused face pass run (lam' in FP32
+// with lam32), else the 19 cotangents of every face.
+int adjoint_record_format(const Ctx& c, bool lam32) {
+  if (!gather_z_supported(c) || !adjoint_faces_fused_covers(c)) return kFacesFull;
+  return lam32 ? kFacesCompactLam32 : kFacesCompact;
 }
 
 bool adjoint_flow_fused(const Ctx& c) { return c.precision == 1 && adjoint_lam32_supported(c); }

@@ -941,7 +941,7 @@ int dugks_forward_record(dugks_ctx* ctx, int istep0, int T, int loss_kind, int i
   // and the controller's previous-valid flag when the record begins
   const int ax = c.s.stream_axis();
   if (flow) {
-    const int fmt = c.precision == 0 ? kFacesFull : kFacesCompactLam32;
+    const int fmt = adjoint_record_format(c, c.precision != 0);
     if ((rc = chk(pool_doubles(c, w.frec, w.frec_len, (size_t)T + 1), "force record alloc"))) return rc;
     size_t ctl_len = w.ctl ? (size_t)kFlowCtlLen : 0;
     if ((rc = chk(pool_doubles(c, w.ctl, ctl_len, kFlowCtlLen), "controller scalars alloc"))) return rc;

@@ -57,7 +57,8 @@ hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled)
 // Adjoint-face pass of the fused kernel (see adjoint.hip); handled = false
 // when the fused kernel does not cover the configuration.  Face record
 // formats: kFacesFull, the 19 reconstructed-value cotangents per face, lam'
-// in FP64; kFacesCompact (FP32 arithmetic only), kCompact values per face:
+// in FP64; kFacesCompact (the z-marching gather's configurations, FP32 or
+// FP64 arithmetic; adjoint_record_format), kCompact values per face:
 // the reconstructed-value cotangents of the ten populations crossing the face
 // (fw1 mu + Ac + B.c), then Ac and the two tangential components of B, from
 // which the nine others follow; kFacesCompactLam32, the same with lam' in FP32.
@@ -115,6 +116,9 @@ hipError_t launch_adjoint_lam32(Ctx& c, const void* f, const float* lam_next, fl
 // lam' as it reads it (FP32 storage with FP32 lam) instead of a pass of its own.
 bool adjoint_flow_fused(const Ctx& c);
 bool adjoint_lam32_supported(const Ctx& c);
+// The face-record format (kFaces*) of an adjoint step of this context, lam'
+// in FP32 (lam32) or FP64.
+int adjoint_record_format(const Ctx& c, bool lam32);
 // Whether the adjoint covers the flow-rate controller (the ambient-force
 // cotangent of the face collisions is available).
 bool adjoint_flow_supported(const Ctx& c);

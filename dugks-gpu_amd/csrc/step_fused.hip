@@ -995,17 +995,20 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, vo
                                       const double* ubar) {
   int tx = 32, ty = 4;
   *handled = false;
-  if (!adjoint_faces_fused_covers(c) || (format != kFacesFull && c.precision == 0)) return hipSuccess;
+  if (!adjoint_faces_fused_covers(c) || (format == kFacesCompactLam32 && c.precision == 0)) return hipSuccess;
   fused_supported(c, &tx, &ty);
   *handled = true;
-  // Compact records go with the z-marching gather (nx % 32 == 0): there the
-  // pass runs on 32 x 4 tiles, whose face rows are whole 128-byte lines
-  // (16-wide rows write half lines; on the 256x192x256 channel the adjoint
-  // step with FP32 lam took 2.92 ms with 16 x 12 tiles against 2.76 despite
-  // the smaller halo).  With FP64 lam 16 x 12 tiles were faster (3.30
-  // against 3.47 ms), but both lam forms keep the same tiles: the same
-  // kernel code then gives the same bits for a lam' that FP32 holds exactly.
-  const bool tall = tx == 16 && format == kFacesFull;
+  // Compact records go with the z-marching gather (nx % 32 == 0).  With FP32
+  // arithmetic the pass runs on 32 x 4 tiles, whose face rows are whole
+  // 128-byte lines (16-wide rows write half lines; on the 256x192x256
+  // channel the adjoint step with FP32 lam took 2.92 ms with 16 x 12 tiles
+  // against 2.76 despite the smaller halo).  With FP64 lam 16 x 12 tiles
+  // were faster (3.30 against 3.47 ms), but both lam forms keep the same
+  // tiles: the same kernel code then gives the same bits for a lam' that
+  // FP32 holds exactly.  With FP64 arithmetic 16-wide rows of doubles are
+  // whole lines, and the compact records take 16 x 12 tiles where the grid
+  // divides (FP64 adjoint step 5.88 against 6.61 ms on 32 x 4).
+  const bool tall = tx == 16 && (format == kFacesFull || c.precision == 0);
   if (fpart) {
     // with the ambient-force cotangent: the formats of the forward-adjoint
     // sweep (FP64, and FP32 arithmetic with lam in FP32)
@@ -1014,6 +1017,11 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, vo
                                                                                 nparts)
                   : run_adj_faces<double, double, 32, 4, false, double, true>(c, f, lam, faces, fpart, fpart_len,
                                                                                nparts);
+    if (c.precision == 0 && format == kFacesCompact)
+      return tall ? run_adj_faces<double, double, 16, 12, true, double, true>(c, f, lam, faces, fpart, fpart_len,
+                                                                               nparts)
+                  : run_adj_faces<double, double, 32, 4, true, double, true>(c, f, lam, faces, fpart, fpart_len,
+                                                                              nparts);
     if (c.precision == 1 && format == kFacesCompactLam32 && !tall)
       return run_adj_faces<float, float, 32, 4, true, float, true>(c, f, lam, faces, fpart, fpart_len, nparts, mom,
                                                                    ubar);
@@ -1023,6 +1031,9 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, vo
   }
   switch (c.precision) {
     case 0:
+      if (format == kFacesCompact)
+        return tall ? run_adj_faces<double, double, 16, 12, true>(c, f, lam, faces)
+                    : run_adj_faces<double, double, 32, 4, true>(c, f, lam, faces);
       return tall ? run_adj_faces<double, double, 16, 12>(c, f, lam, faces)
                   : run_adj_faces<double, double, 32, 4>(c, f, lam, faces);
     case 1:
@@ -1049,7 +1060,8 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, vo
 long long adjoint_faces_fused_blocks(const Ctx& c, int format) {
   int tx = 32, ty = 4, TZ = 0;
   if (!fused_supported(c, &tx, &ty)) return 0;
-  const dim3 g = tx == 16 && format == kFacesFull ? adj_faces_grid<16, 12>(c, &TZ) : adj_faces_grid<32, 4>(c, &TZ);
+  const bool tall = tx == 16 && (format == kFacesFull || c.precision == 0);
+  const dim3 g = tall ? adj_faces_grid<16, 12>(c, &TZ) : adj_faces_grid<32, 4>(c, &TZ);
   return (long long)g.x * g.y * g.z;
 }
 
