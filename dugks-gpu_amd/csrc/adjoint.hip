@@ -342,16 +342,17 @@ __global__ void __launch_bounds__(256) k_adj_cells(KParams P, const Store* __res
 // in-plane gather at the ghost position, which takes the same z transposes
 // as its own.  Each face value is read from HBM about 1.3 times (the ring)
 // instead of about 2.5 times.  The centre-stage adjoint needs only rho and j
-// of the state: with FP32 storage it reads them from the face pass (16 bytes
-// a cell) rather than the 19 values of f.
+// of the state: with the state stored in the arithmetic type it reads them
+// from the face pass (16 or 32 bytes a cell) rather than the 19 values of f.
 constexpr int kGX = 32, kGY = 8;
 constexpr int kGW = kGX + 2;          // staged row length
 constexpr int kGP = (kGY + 2) * kGW;  // staged positions per array and population
 constexpr int kGT = kGX * kGY;        // lanes: one per cell column
 static_assert(kGP <= 2 * kGT, "two staged positions per lane at most");
 
-// mom: the region of rho, j behind the face records that holds fin's (FP32
-// storage).  FLOW (FP32 storage, FP32 lam; a sweep through the flow-rate
+// mom: the region of rho, j behind the face records that holds fin's
+// (storage in the arithmetic type).  FLOW (such storage, with FP32 lam and
+// arithmetic or FP64 throughout; a sweep through the flow-rate
 // controller): lam' still lacks the bulk velocity's cotangent ubar (dV/V)
 // (c_ia - u_a)/rho of f(n+1), whose rho, j the previous pass left in the
 // other region; it is added as lam' is used.
@@ -557,9 +558,10 @@ __global__ void __launch_bounds__(kGT) k_adj_cells_z(KParams P, const Store* __r
   const unsigned Nb = (unsigned)(P.N * (long long)sizeof(Store));
   const RawBuf lamb(lam, (unsigned)(Q * P.N * sizeof(Lam)));
   const unsigned lam_pop = (unsigned)(P.N * sizeof(Lam));
-  // with FP32 storage the face pass has stored rho and j of every cell
-  // behind the records: 16 bytes read per cell instead of 76 of f
-  constexpr bool kMoments = std::is_same<Store, float>::value;
+  // with the state stored in the arithmetic type the face pass has stored
+  // rho and j of every cell behind the records: 4 values read per cell
+  // instead of 19 of f
+  constexpr bool kMoments = std::is_same<Store, Real>::value;
   const RawBuf mb(faces + 3LL * kCompact * P.eN + mom * 4LL * P.N, (unsigned)(4 * P.N * sizeof(Real)));
   const unsigned mom_pop = (unsigned)(P.N * sizeof(Real));
   // FLOW: rho and j_a of f(n+1) at the cell, and s0 = ubar dV / V
@@ -731,8 +733,10 @@ hipError_t run_adjoint(Ctx& cx, const void* f, const Lam* lam_next, Lam* lam_pre
     if (e0 != hipSuccess) return e0;
     cx.adj_faces_format = fmt;
   }
-  // the controller term rides on lam' only on the FP32-storage, FP32-lam form
-  constexpr bool kFuse = std::is_same<Store, float>::value && kLam32;
+  // the controller term rides on lam' where the face pass hands rho and j to
+  // the gather (storage in the arithmetic type) and lam' is the sweep's:
+  // FP32 with FP32 arithmetic, FP64 with FP64
+  constexpr bool kFuse = std::is_same<Store, Real>::value && (kLam32 || std::is_same<Real, double>::value);
   if (ubar && !kFuse) return hipErrorNotSupported;
   hipError_t e = launch_adjoint_faces_fused(cx, f, lam_next, faces, format, &handled, fpart, fpart_len, nparts, mom,
                                             ubar);
@@ -835,8 +839,8 @@ hipError_t adjoint_step(Ctx& c, const void* f, const Lam* lam_next, Lam* lam_pre
 }  // namespace
 
 hipError_t launch_adjoint(Ctx& c, const void* f, const double* lam_next, double* lam_prev, double* grad_force,
-                          double* fpart, long long fpart_len, long long* nparts) {
-  return adjoint_step<double>(c, f, lam_next, lam_prev, grad_force, fpart, fpart_len, nparts, 0, nullptr);
+                          double* fpart, long long fpart_len, long long* nparts, int mom, const double* ubar) {
+  return adjoint_step<double>(c, f, lam_next, lam_prev, grad_force, fpart, fpart_len, nparts, mom, ubar);
 }
 
 hipError_t launch_adjoint_lam32(Ctx& c, const void* f, const float* lam_next, float* lam_prev, double* grad_force,
@@ -853,7 +857,10 @@ int adjoint_record_format(const Ctx& c, bool lam32) {
   return lam32 ? kFacesCompactLam32 : kFacesCompact;
 }
 
-bool adjoint_flow_fused(const Ctx& c) { return c.precision == 1 && adjoint_lam32_supported(c); }
+bool adjoint_flow_fused(const Ctx& c) {
+  if (c.precision == 0) return adjoint_record_format(c, false) == kFacesCompact;
+  return c.precision == 1 && adjoint_lam32_supported(c);
+}
 
 bool adjoint_lam32_supported(const Ctx& c) {
   if (c.precision == 0 || c.precision == 3 || c.s.recon != 0) return false;

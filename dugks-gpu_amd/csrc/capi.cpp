@@ -1042,7 +1042,7 @@ int dugks_adjoint_replay(dugks_ctx* ctx, double* dev_grad_force, double* dev_gra
   // step also yields the face part of the force cotangent phi_n, and the
   // change of the summed cell force gradient gives the rest.
   const bool flow = w.flow;
-  const bool fused = flow && lam32 && adjoint_flow_fused(c);
+  const bool fused = flow && adjoint_flow_fused(c);
   const int ax = c.s.stream_axis();
   std::unique_ptr<FrozenFlow> frozen;
   long long nparts = 0;
@@ -1098,7 +1098,8 @@ int dugks_adjoint_replay(dugks_ctx* ctx, double* dev_grad_force, double* dev_gra
                                            dev_grad_force, w.fpart, (long long)w.fpart_len, &nparts,
                                            fused ? (n & 1) : 0, ride ? w.ctl + kFlowCtlUbar : nullptr)
                     : launch_adjoint(c, state(n), static_cast<const double*>(lam), static_cast<double*>(lam2),
-                                     dev_grad_force, w.fpart, (long long)w.fpart_len, &nparts);
+                                     dev_grad_force, w.fpart, (long long)w.fpart_len, &nparts,
+                                     fused ? (n & 1) : 0, ride ? w.ctl + kFlowCtlUbar : nullptr);
         if ((rc = chk(e, "controller adjoint"))) return rc;
       } else if (lam32) {
         float *l = static_cast<float*>(lam), *l2 = static_cast<float*>(lam2);

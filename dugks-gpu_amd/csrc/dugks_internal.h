@@ -62,20 +62,20 @@ hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled)
 // the reconstructed-value cotangents of the ten populations crossing the face
 // (fw1 mu + Ac + B.c), then Ac and the two tangential components of B, from
 // which the nine others follow; kFacesCompactLam32, the same with lam' in FP32.
-// With FP32 storage the compact formats also carry, after the three record
-// arrays (at 3 kCompact eN values), rho, j_x, j_y, j_z of every cell ([4][N]),
+// With the state stored in the arithmetic type (FP32 or FP64 storage) the
+// compact formats also carry, after the three record arrays (at 3 kCompact eN
+// values), two regions of rho, j_x, j_y, j_z of every cell ([4][N] each),
 // which the gather uses instead of reading f.
 constexpr int kCompact = 13;
 constexpr int kFacesFull = 0, kFacesCompact = 1, kFacesCompactLam32 = 2;
 // With fpart set, the pass also writes per block the cotangent of the
 // ambient force along the flow axis through the face collisions (nparts
-// values, at most fpart_len, else hipErrorInvalidValue); only FP64 full
-// records and FP32-lam compact records have this form, else
-// hipErrorNotSupported.
-// mom (compact records, FP32 storage): which of the two rho, j regions
-// behind the records the pass writes; ubar (FP32-lam sweep through the
-// flow-rate controller, with fpart): the bulk-velocity cotangent to add to
-// lam' from the other region (see AdjFaceIO).
+// values, at most fpart_len, else hipErrorInvalidValue); only FP64 records
+// and FP32-lam compact records have this form, else hipErrorNotSupported.
+// mom (compact records, storage in the arithmetic type): which of the two
+// rho, j regions behind the records the pass writes; ubar (a sweep through
+// the flow-rate controller on those records, with fpart): the bulk-velocity
+// cotangent to add to lam' from the other region (see AdjFaceIO).
 hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, void* faces, int format,
                                       bool* handled, double* fpart = nullptr, long long fpart_len = 0,
                                       long long* nparts = nullptr, int mom = 0, const double* ubar = nullptr);
@@ -104,16 +104,17 @@ hipError_t launch_perturb_force(Ctx& c, int step);
 // cotangent); hipErrorNotSupported where the fused face pass does not run.
 hipError_t launch_adjoint(Ctx& c, const void* f, const double* lam_next, double* lam_prev,
                           double* grad_force, double* fpart = nullptr, long long fpart_len = 0,
-                          long long* nparts = nullptr);
+                          long long* nparts = nullptr, int mom = 0, const double* ubar = nullptr);
 // The same with lam' and lam in FP32; only FP32 arithmetic on the z-marching
 // gather's configurations (adjoint_lam32_supported), else hipErrorNotSupported.
 // mom, ubar: as launch_adjoint_faces_fused; with ubar the gather adds the
-// same controller term (FP32 storage only: adjoint_flow_fused).
+// same controller term (adjoint_flow_fused).
 hipError_t launch_adjoint_lam32(Ctx& c, const void* f, const float* lam_next, float* lam_prev,
                                 double* grad_force, double* fpart = nullptr, long long fpart_len = 0,
                                 long long* nparts = nullptr, int mom = 0, const double* ubar = nullptr);
 // Whether the sweep's adjoint step adds the flow-rate controller's term to
-// lam' as it reads it (FP32 storage with FP32 lam) instead of a pass of its own.
+// lam' as it reads it (compact records with FP32 storage and FP32 lam, or
+// FP64 throughout) instead of a pass of its own.
 bool adjoint_flow_fused(const Ctx& c);
 bool adjoint_lam32_supported(const Ctx& c);
 // The face-record format (kFaces*) of an adjoint step of this context, lam'

@@ -49,10 +49,10 @@ struct Tile {
 struct AdjFaceIO {
   const void* lam;    // lam' = dJ/df(n+1), SoA [19][nz][ny][nx], FP64 or (LamT = float) FP32
   void* faces;        // out: [3][19][ghost-inclusive face slot], arithmetic type; COMPACT:
-                      // [3][kCompact][slot], then (FP32 storage) two [4][N] regions of
-                      // rho, j of every cell
-  int mom = 0;        // COMPACT, FP32 storage: the region the pass writes rho, j of fin into
-  // FP32 sweep through the flow-rate controller: lam' still lacks the bulk
+                      // [3][kCompact][slot], then (storage in the arithmetic type) two
+                      // [4][N] regions of rho, j of every cell
+  int mom = 0;        // COMPACT, such storage: the region the pass writes rho, j of fin into
+  // sweep through the flow-rate controller: lam' still lacks the bulk
   // velocity's cotangent ubar (dV/V) (c_ia - u_a)/rho of f(n+1), whose rho, j
   // the previous pass left in the other region; added as lam' is loaded
   // (nullptr: lam' is complete)
@@ -339,7 +339,7 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
   // the flow-rate controller's term of lam' (AdjFaceIO::ubar): rho and j_a
   // of f(n+1) at the column's cell of plane k, loaded with lam', added before
   // lam' is used: lam'_i += s (c_ia - u_a), s = ubar (dV/V) / rho
-  constexpr bool kFlowFuse = ADJ && FADJ && COMPACT && std::is_same<Store, float>::value;
+  constexpr bool kFlowFuse = ADJ && FADJ && COMPACT && std::is_same<Store, Real>::value;
   const bool fuse = kFlowFuse && adj.ubar != nullptr;
   const RawBuf momn(static_cast<Real*>(adj.faces) + 3LL * kCompact * P.eN + (1 - adj.mom) * 4LL * P.N,
                     (unsigned)(4 * P.N * sizeof(Real)));
@@ -481,7 +481,8 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
   }
 
   if constexpr (kPrefetch) fpre.settle();
-  // compact adjoint-face records with FP32 storage: rho and j of every cell
+  // compact adjoint-face records with the state stored in the arithmetic
+  // type (FP32 or FP64): rho and j of every cell
   // for the gather, written by the interior lanes from the prefetched f.
   // A chunk writes its planes z0+1 .. zend (mod nz), so the chunks cover
   // every plane once.
@@ -516,7 +517,7 @@ __global__ void __launch_bounds__(((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64,
         centre_fbar_shift<Real>(ff, rho, u0, F, hdt, (Real)P.bw0, (Real)P.bw2, fbn);
       } else {
         fpre.decode(P, ff, rho, u0);
-        if constexpr (ADJ && COMPACT && std::is_same<Store, float>::value) {
+        if constexpr (ADJ && COMPACT && std::is_same<Store, Real>::value) {
           if (interior) {
             Real jm[3];
             cell_moments<Real>(ff, rho, jm);
@@ -1019,9 +1020,9 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, vo
                                                                                nparts);
     if (c.precision == 0 && format == kFacesCompact)
       return tall ? run_adj_faces<double, double, 16, 12, true, double, true>(c, f, lam, faces, fpart, fpart_len,
-                                                                               nparts)
+                                                                               nparts, mom, ubar)
                   : run_adj_faces<double, double, 32, 4, true, double, true>(c, f, lam, faces, fpart, fpart_len,
-                                                                              nparts);
+                                                                              nparts, mom, ubar);
     if (c.precision == 1 && format == kFacesCompactLam32 && !tall)
       return run_adj_faces<float, float, 32, 4, true, float, true>(c, f, lam, faces, fpart, fpart_len, nparts, mom,
                                                                    ubar);
@@ -1032,8 +1033,8 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, vo
   switch (c.precision) {
     case 0:
       if (format == kFacesCompact)
-        return tall ? run_adj_faces<double, double, 16, 12, true>(c, f, lam, faces)
-                    : run_adj_faces<double, double, 32, 4, true>(c, f, lam, faces);
+        return tall ? run_adj_faces<double, double, 16, 12, true>(c, f, lam, faces, nullptr, 0, nullptr, mom)
+                    : run_adj_faces<double, double, 32, 4, true>(c, f, lam, faces, nullptr, 0, nullptr, mom);
       return tall ? run_adj_faces<double, double, 16, 12>(c, f, lam, faces)
                   : run_adj_faces<double, double, 32, 4>(c, f, lam, faces);
     case 1:
