@@ -176,9 +176,9 @@ def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32"):
     S = {"fp64": 8, "fp32": 4}[precision]
     lb = 4 if lam_type == "float32" else 8
     rec = 2 * 3 * 13 * S  # compact records (the z-marching gather's configurations)
-    # FP32 storage: the face pass hands rho and j (16 B) to the gather, which
-    # then does not read f
-    state = 19 * S + 2 * 16 if precision == "fp32" else 2 * 19 * S
+    # storage in the arithmetic type: the face pass hands rho and j (4 values)
+    # to the gather, which then does not read f
+    state = 19 * S + 2 * 4 * S
     adj_bytes = eng.ncell * (3 * 19 * lb + state + rec + 2 * 3 * 8)
     eng.set_f_device(f0)
     del f0
@@ -201,9 +201,8 @@ def adjoint_leg(dugks_amd, configs, torch, T, precision="fp32"):
                adjoint_over_forward_step=round(t_adj[lam_type] / fwd_step, 3),
                adjoint_step_ms_fp64_lambda=round(t_adj["float64"] * 1e3, 4),
                # algorithmic bytes of one adjoint step (per cell: lam' read twice and lam
-               # written, f read once and rho, j written and read (FP32; else f read
-               # twice), 13-value face records of 3 axes written and read, the FP64
-               # force gradient read and written)
+               # written, f read once and rho, j written and read, 13-value face records
+               # of 3 axes written and read, the FP64 force gradient read and written)
                adjoint_alg_bytes_per_step=adj_bytes,
                adjoint_achieved_gbs=round(adj_bytes / t_adj[lam_type] / 1e9, 1),
                fwd_adj_sweep_s=round(t_fa, 3),
