@@ -991,6 +991,26 @@ bool adjoint_faces_fused_covers(const Ctx& c) {
   return (double)Q * c.kp.N * 8.0 < 2147483648.0 && (double)Q * c.kp.eN * rb < 2147483648.0;
 }
 
+namespace {
+// Tiles of the adjoint-face pass: 0 = 32 x 4, 1 = 16 x 12, 2 = 32 x 12.
+// Compact records go with the z-marching gather (nx % 32 == 0).  With FP32
+// arithmetic their rows must be whole 128-byte lines, so 32-wide: 16 x 12
+// tiles (half-line stores) took the adjoint step with FP32 lam to 2.92 ms on
+// the 256x192x256 channel against 2.76 on 32 x 4, and 32 x 12 tiles (one
+// workgroup of 8 waves per CU, f-bar+ on 1.24x the cells instead of 1.59x,
+// 93 % of the lanes busy) bring it to 2.60 where ny % 12 == 0.  Both lam
+// forms take the same tiles, so the same kernel code gives the same bits
+// for a lam' that FP32 holds exactly.  With FP64 arithmetic 16 doubles are a
+// whole line, and 16 x 12 tiles serve every format where the grid divides
+// (FP64 adjoint step 5.88 against 6.61 ms on 32 x 4).  FP32 full records
+// take the step's tiles.
+int adj_tile(const Ctx& c, int format, int tx) {
+  const bool compact32 = c.precision != 0 && format != kFacesFull;
+  if (compact32) return c.kp.n[1] % 12 == 0 ? 2 : 0;
+  return tx == 16 ? 1 : 0;
+}
+}  // namespace
+
 hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, void* faces, int format,
                                       bool* handled, double* fpart, long long fpart_len, long long* nparts, int mom,
                                       const double* ubar) {
@@ -999,17 +1019,8 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, vo
   if (!adjoint_faces_fused_covers(c) || (format == kFacesCompactLam32 && c.precision == 0)) return hipSuccess;
   fused_supported(c, &tx, &ty);
   *handled = true;
-  // Compact records go with the z-marching gather (nx % 32 == 0).  With FP32
-  // arithmetic the pass runs on 32 x 4 tiles, whose face rows are whole
-  // 128-byte lines (16-wide rows write half lines; on the 256x192x256
-  // channel the adjoint step with FP32 lam took 2.92 ms with 16 x 12 tiles
-  // against 2.76 despite the smaller halo).  With FP64 lam 16 x 12 tiles
-  // were faster (3.30 against 3.47 ms), but both lam forms keep the same
-  // tiles: the same kernel code then gives the same bits for a lam' that
-  // FP32 holds exactly.  With FP64 arithmetic 16-wide rows of doubles are
-  // whole lines, and the compact records take 16 x 12 tiles where the grid
-  // divides (FP64 adjoint step 5.88 against 6.61 ms on 32 x 4).
-  const bool tall = tx == 16 && (format == kFacesFull || c.precision == 0);
+  const int tile = adj_tile(c, format, tx);
+  const bool tall = tile == 1, wide = tile == 2;
   if (fpart) {
     // with the ambient-force cotangent: the formats of the forward-adjoint
     // sweep (FP64, and FP32 arithmetic with lam in FP32)
@@ -1023,11 +1034,16 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, vo
                                                                                nparts, mom, ubar)
                   : run_adj_faces<double, double, 32, 4, true, double, true>(c, f, lam, faces, fpart, fpart_len,
                                                                               nparts, mom, ubar);
-    if (c.precision == 1 && format == kFacesCompactLam32 && !tall)
-      return run_adj_faces<float, float, 32, 4, true, float, true>(c, f, lam, faces, fpart, fpart_len, nparts, mom,
-                                                                   ubar);
-    if (c.precision == 2 && format == kFacesCompactLam32 && !tall)
-      return run_adj_faces<float, __half, 32, 4, true, float, true>(c, f, lam, faces, fpart, fpart_len, nparts);
+    if (c.precision == 1 && format == kFacesCompactLam32)
+      return wide ? run_adj_faces<float, float, 32, 12, true, float, true>(c, f, lam, faces, fpart, fpart_len,
+                                                                            nparts, mom, ubar)
+                  : run_adj_faces<float, float, 32, 4, true, float, true>(c, f, lam, faces, fpart, fpart_len,
+                                                                           nparts, mom, ubar);
+    if (c.precision == 2 && format == kFacesCompactLam32)
+      return wide ? run_adj_faces<float, __half, 32, 12, true, float, true>(c, f, lam, faces, fpart, fpart_len,
+                                                                             nparts)
+                  : run_adj_faces<float, __half, 32, 4, true, float, true>(c, f, lam, faces, fpart, fpart_len,
+                                                                            nparts);
     return hipErrorNotSupported;
   }
   switch (c.precision) {
@@ -1039,19 +1055,19 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, vo
                   : run_adj_faces<double, double, 32, 4>(c, f, lam, faces);
     case 1:
       if (format == kFacesCompactLam32)
-        return tall ? run_adj_faces<float, float, 16, 12, true, float>(c, f, lam, faces, nullptr, 0, nullptr, mom)
+        return wide ? run_adj_faces<float, float, 32, 12, true, float>(c, f, lam, faces, nullptr, 0, nullptr, mom)
                     : run_adj_faces<float, float, 32, 4, true, float>(c, f, lam, faces, nullptr, 0, nullptr, mom);
       if (format == kFacesCompact)
-        return tall ? run_adj_faces<float, float, 16, 12, true>(c, f, lam, faces, nullptr, 0, nullptr, mom)
+        return wide ? run_adj_faces<float, float, 32, 12, true>(c, f, lam, faces, nullptr, 0, nullptr, mom)
                     : run_adj_faces<float, float, 32, 4, true>(c, f, lam, faces, nullptr, 0, nullptr, mom);
       return tall ? run_adj_faces<float, float, 16, 12>(c, f, lam, faces)
                   : run_adj_faces<float, float, 32, 4>(c, f, lam, faces);
     default:
       if (format == kFacesCompactLam32)
-        return tall ? run_adj_faces<float, __half, 16, 12, true, float>(c, f, lam, faces)
+        return wide ? run_adj_faces<float, __half, 32, 12, true, float>(c, f, lam, faces)
                     : run_adj_faces<float, __half, 32, 4, true, float>(c, f, lam, faces);
       if (format == kFacesCompact)
-        return tall ? run_adj_faces<float, __half, 16, 12, true>(c, f, lam, faces)
+        return wide ? run_adj_faces<float, __half, 32, 12, true>(c, f, lam, faces)
                     : run_adj_faces<float, __half, 32, 4, true>(c, f, lam, faces);
       return tall ? run_adj_faces<float, __half, 16, 12>(c, f, lam, faces)
                   : run_adj_faces<float, __half, 32, 4>(c, f, lam, faces);
@@ -1061,8 +1077,9 @@ hipError_t launch_adjoint_faces_fused(Ctx& c, const void* f, const void* lam, vo
 long long adjoint_faces_fused_blocks(const Ctx& c, int format) {
   int tx = 32, ty = 4, TZ = 0;
   if (!fused_supported(c, &tx, &ty)) return 0;
-  const bool tall = tx == 16 && (format == kFacesFull || c.precision == 0);
-  const dim3 g = tall ? adj_faces_grid<16, 12>(c, &TZ) : adj_faces_grid<32, 4>(c, &TZ);
+  const int tile = adj_tile(c, format, tx);
+  const dim3 g = tile == 1 ? adj_faces_grid<16, 12>(c, &TZ)
+                           : (tile == 2 ? adj_faces_grid<32, 12>(c, &TZ) : adj_faces_grid<32, 4>(c, &TZ));
   return (long long)g.x * g.y * g.z;
 }
 
