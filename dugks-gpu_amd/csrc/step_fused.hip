@@ -977,6 +977,15 @@ hipError_t launch_step_fused(Ctx& c, const void* fin, void* fout, bool* handled)
   *handled = false;
   if (!fused_supported(c, &tx, &ty)) return hipSuccess;
   *handled = true;
+  // FP32 storage takes 32 x 12 tiles where the grid divides: one workgroup
+  // of 8 waves per CU, rows of 32 floats that are whole 128-byte lines for
+  // every load and store (16-wide rows store half lines), f-bar+ on 1.24x the
+  // cells.  On the 256x192x256 channel 0.751 ms against 0.819 on 16 x 12;
+  // the FP16 storage modes lost with them (non-equilibrium 1.080 against
+  // 0.967 ms, deviation 0.809 against 0.771) and FP64 would need 210 KB of
+  // LDS.
+  if (c.precision == 1 && c.s.recon == 0 && c.kp.n[0] % 32 == 0 && c.kp.n[1] % 12 == 0)
+    return run_fused<float, float, 32, 12, false>(c, fin, fout);
   return c.s.recon == 0 ? launch_fused_recon<false>(c, fin, fout, tx == 16)
                         : launch_fused_recon<true>(c, fin, fout, tx == 16);
 }

@@ -166,7 +166,8 @@ def test_cell_force_matches_oracle(rng, kernel):
 
 
 @pytest.mark.parametrize("kernel", KERNELS, ids=["twopass", "fused"])
-@pytest.mark.parametrize("nx", [32, 64])
+# fused tiles: 32 x 4 (ny = 16), and for FP32 storage 32 x 12 (ny = 24)
+@pytest.mark.parametrize("nx,ny", [(32, 16), (64, 16), (32, 24), (64, 24)])
 # The FP16 modes are checked population by population against the rounding
 # of their codec (tol = None), on top of the FP32 arithmetic of the decode,
 # the step and the encode (twice the FP32 step's bar):
@@ -178,9 +179,9 @@ def test_cell_force_matches_oracle(rng, kernel):
 @pytest.mark.parametrize("precision,tol", [(PREC_FP32, 2e-6), (PREC_FP16S, None), (PREC_FP16N, None)],
                          ids=["fp32", "fp16", "fp16n"])
 @pytest.mark.parametrize("recon", ["tri", "grad"])
-def test_reduced_precision_one_step(rng, precision, tol, kernel, nx, recon):
-    text = configs.base(nx=nx, ny=16, nz=4, periods_y=0, visc=0.02, ambient_fx=2e-4,
-                        xwidth=nx / 10, ywidth=1.6, zwidth=0.4, recon=recon)
+def test_reduced_precision_one_step(rng, precision, tol, kernel, nx, ny, recon):
+    text = configs.base(nx=nx, ny=ny, nz=4, periods_y=0, visc=0.02, ambient_fx=2e-4,
+                        xwidth=nx / 10, ywidth=ny / 10, zwidth=0.4, recon=recon)
     eng = _engine(text, precision=precision, kernel=kernel)
     ora = _oracle_for(eng, text)
     f0 = random_state(eng.shape, rng)
