@@ -36,6 +36,9 @@ namespace {
 
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
 
+// threads of a TX x TY tile's workgroup (Tile<TX, TY>::NT)
+constexpr int Tile_nt(int TX, int TY) { return ((TY + 2) * TX + 2 * (TY + 2) + 63) / 64 * 64; }
+
 template <int TX, int TY>
 struct Tile {
   static constexpr int NROW = (TY + 2) * TX;    // the tile rows and the two y-halo rows
@@ -849,22 +852,27 @@ __global__ void __launch_bounds__(256) k_flow_finish(const double* partial, int 
   }
 }
 
-// z-chunk length: 32 planes unless that leaves fewer tiles than one round of
-// two resident blocks on each of the 256 CUs; then halve it, down to 8.  A
-// chunk recomputes f-bar+ of the two planes before it, so short chunks cost
-// that much more work; more rounds of blocks even out the finish.  On the
-// 256x192x256 channel (256 columns of 16x12, 2048 tiles): FP32 0.823 ms
-// against 0.833 with 64 planes and 0.833 with 16, FP16 deviation storage
-// 0.738 against 0.758 and 0.740 (128 planes: 0.865 / 0.783).  On 128^3
-// (128 columns of 32x4, 512 tiles) 32 planes against the 8 that a rule of
-// eight tiles per CU gives: Taylor-Green FP32 0.148 against 0.166 ms, FP64
-// 0.397 against 0.444 ms; the Re_tau = 180 channel FP32 0.185 against 0.195,
-// FP64 0.426 against 0.472 ms.
+// z-chunk length.  A chunk recomputes f-bar+ of the two planes before it.
+// With two resident workgroups per CU (FP32 arithmetic on 4-wave tiles) the
+// other workgroup covers that prologue, and more rounds of workgroups even
+// out the finish: 32 planes, on the 256x192x256 channel (256 columns of
+// 16x12, 2048 tiles) FP32 0.823 ms against 0.833 with 64 planes and 0.833
+// with 16, FP16 deviation storage 0.738 against 0.758 and 0.740 (128
+// planes: 0.865 / 0.783); on 128^3 (128 columns of 32x4, 512 tiles) 32
+// planes against 8: Taylor-Green FP32 0.148 against 0.166 ms, FP64 0.397
+// against 0.444 ms.  With one workgroup per CU (FP64, and the 8-wave
+// 32x12 tiles) nothing covers it and long chunks win: 128 planes, FP32 on
+// 32x12 0.724 ms against 0.731 with 64 and 0.751 with 32, FP64 on 16x12
+// 1.921 against 1.945 and 1.974.  Either start is halved, down to 8, while
+// the grid has fewer tiles than one round of resident workgroups on the
+// 256 CUs.
+int blocks_per_cu(bool fp64, int TX, int TY) { return fp64 || Tile_nt(TX, TY) > 256 ? 1 : 2; }
 int z_chunk(const Ctx& cx, int TX, int TY) {
   if (cx.fused_tz > 0) return cx.fused_tz;
   const long long cols = (long long)(cx.kp.n[0] / TX) * (cx.kp.n[1] / TY);
-  int tz = 32;
-  while (tz > 8 && cols * ((cx.kp.n[2] + tz - 1) / tz) < 2 * 256) tz /= 2;
+  const int per_cu = blocks_per_cu(cx.precision == 0, TX, TY);
+  int tz = per_cu == 1 ? 128 : 32;
+  while (tz > 8 && cols * ((cx.kp.n[2] + tz - 1) / tz) < per_cu * 256) tz /= 2;
   return tz;
 }
 
