@@ -861,17 +861,18 @@ __global__ void __launch_bounds__(256) k_flow_finish(const double* partial, int 
 // planes: 0.865 / 0.783); on 128^3 (128 columns of 32x4, 512 tiles) 32
 // planes against 8: Taylor-Green FP32 0.148 against 0.166 ms, FP64 0.397
 // against 0.444 ms.  With one workgroup per CU (FP64, and the 8-wave
-// 32x12 tiles) nothing covers it and long chunks win: 128 planes, FP32 on
-// 32x12 0.724 ms against 0.731 with 64 and 0.751 with 32, FP64 on 16x12
-// 1.921 against 1.945 and 1.974.  Either start is halved, down to 8, while
-// the grid has fewer tiles than one round of resident workgroups on the
-// 256 CUs.
+// 32x12 tiles) nothing covers it and long chunks win for the step: 128
+// planes, FP32 on 32x12 0.724 ms against 0.731 with 64 and 0.751 with 32,
+// FP64 on 16x12 1.921 against 1.945 and 1.974.  The adjoint-face passes
+// (adj) keep 32 planes (with 128 their adjoint steps were 0.7 % slower).
+// Either start is halved, down to 8, while the grid has fewer tiles than one
+// round of resident workgroups on the 256 CUs.
 int blocks_per_cu(bool fp64, int TX, int TY) { return fp64 || Tile_nt(TX, TY) > 256 ? 1 : 2; }
-int z_chunk(const Ctx& cx, int TX, int TY) {
+int z_chunk(const Ctx& cx, int TX, int TY, bool adj = false) {
   if (cx.fused_tz > 0) return cx.fused_tz;
   const long long cols = (long long)(cx.kp.n[0] / TX) * (cx.kp.n[1] / TY);
   const int per_cu = blocks_per_cu(cx.precision == 0, TX, TY);
-  int tz = per_cu == 1 ? 128 : 32;
+  int tz = per_cu == 1 && !adj ? 128 : 32;
   while (tz > 8 && cols * ((cx.kp.n[2] + tz - 1) / tz) < per_cu * 256) tz /= 2;
   return tz;
 }
@@ -907,7 +908,7 @@ hipError_t run_fused(Ctx& cx, const void* fin, void* fout) {
 template <int TX, int TY>
 dim3 adj_faces_grid(const Ctx& cx, int* TZ) {
   const KParams& P = cx.kp;
-  *TZ = z_chunk(cx, TX, TY);
+  *TZ = z_chunk(cx, TX, TY, true);
   return dim3(P.n[0] / TX, P.n[1] / TY, (P.n[2] + *TZ - 1) / *TZ);
 }
 
